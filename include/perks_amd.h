@@ -1,0 +1,186 @@
+/*
+ * perks_amd.h -- C ABI of the MI355X (gfx950) persistent Jacobi stencil / CG library.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no HIP or torch
+ * types.  `stream` arguments take a hipStream_t cast to void* (NULL = the
+ * default stream).  Every function returns PERKS_OK (0) or a negative error
+ * code; nothing in the library calls exit().
+ *
+ * Reference interfaces these entry points replace (paths in the PERKS tree):
+ *   perks_j3d_iterative_{f64,f32}      template<class REAL> int j3d_iterative(...)
+ *                                      stencil/3dstencil/common/jacobi_cuda.cuh:7-8,
+ *                                      defined stencil/3dstencil/j3d.cu:129-808
+ *   perks_j3d_min_height               j3d_iterative(..., getminHeight=true)
+ *                                      stencil/3dstencil/j3d.cu:598,612
+ *   perks_get_min_width_y              template<class REAL> int getMinWidthY(int,int,int,bool)
+ *                                      stencil/3dstencil/common/jacobi_cuda.cuh:12; j3d.cu:69-123
+ *   perks_jacobi2d_iterative_{f64,f32} template<class REAL> int jacobi_iterative(...)
+ *                                      stencil/2dstencil/common/jacobi_cuda.cuh:6-10;
+ *                                      stencil/2dstencil/jacobi-2d.cu:369-865
+ *   perks_cg_solve_f64                 myTest<double,int,...> + DispatchCG::ProcessDispatchCG
+ *                                      conjugateGradient/cg_driver.cuh:4-5; cg_driver.cu:53-428;
+ *                                      conjugateGradient/cg_dispatcher.cuh:559-916
+ * The reference's C++ template signatures are reproduced on top of this ABI
+ * in include/perks_amd/jacobi_cuda.hpp.
+ */
+#ifndef PERKS_AMD_H
+#define PERKS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (negative values keep the reference planner's meaning) -- */
+#define PERKS_OK 0
+#define PERKS_ERR_NO_LDS_FOR_CACHE (-1) /* j3d.cu:591  useSM but no LDS plane fits      */
+#define PERKS_ERR_UNSUPPORTED_SIZE (-2) /* j3d.driver.cpp:186  x/y widths not supported */
+#define PERKS_ERR_NOT_LAUNCHABLE (-3)   /* j3d.cu:570  occupancy is 0                   */
+#define PERKS_ERR_HEIGHT_TOO_SMALL (-4) /* j3d.cu:606  gridDim.z*(2H+1) > height         */
+#define PERKS_ERR_SPILL (-5)            /* j3d.cu:482  cache depth would spill          */
+#define PERKS_ERR_INVALID_ARG (-10)
+#define PERKS_ERR_HIP (-11)            /* a HIP runtime call failed                    */
+#define PERKS_ERR_NO_DEVICE (-12)
+#define PERKS_ERR_UNSUPPORTED (-13)     /* shape/dtype/method combination not built     */
+#define PERKS_ERR_NOT_CONVERGED (-14)
+
+/* ---- enums ---------------------------------------------------------------- */
+enum perks_dtype { PERKS_F64 = 0, PERKS_F32 = 1, PERKS_F16 = 2, PERKS_BF16 = 3 };
+
+/* Shapes are named by (dimension, kind, radius).  Reference directory names:
+ * 3d7pt, 3d13pt, 3d17pt, 3d27pt, poisson; 2d5pt, 2ds9pt, 2d13pt, 2d17pt,
+ * 2d21pt, 2ds25pt (stars r1..r6); 2d9pt, 2d25pt (boxes r1, r2). */
+enum perks_shape {
+    PERKS_SHAPE_3D_STAR_R1 = 0,
+    PERKS_SHAPE_3D_STAR_R2 = 1,
+    PERKS_SHAPE_3D_TYPE0_R1 = 2,
+    PERKS_SHAPE_3D_BOX_R1 = 3,
+    PERKS_SHAPE_3D_POISSON_R1 = 4,
+    PERKS_SHAPE_2D_STAR_R1 = 10,
+    PERKS_SHAPE_2D_STAR_R2 = 11,
+    PERKS_SHAPE_2D_STAR_R3 = 12,
+    PERKS_SHAPE_2D_STAR_R4 = 13,
+    PERKS_SHAPE_2D_STAR_R5 = 14,
+    PERKS_SHAPE_2D_STAR_R6 = 15,
+    PERKS_SHAPE_2D_BOX_R1 = 16,
+    PERKS_SHAPE_2D_BOX_R2 = 17
+};
+
+/* Boundary: CLAMP replicates the edge cell and updates every cell (the
+ * reference's rule, 3d7pt_gold_gen.cpp:21-26).  ZERO treats the array as the
+ * interior of a grid whose one-cell (radius-cell) halo is fixed at 0. */
+enum perks_boundary { PERKS_BC_CLAMP = 0, PERKS_BC_ZERO = 1 };
+
+/* PLAIN: one kernel launch per time step (the reference's kernel3d_baseline).
+ * PERSISTENT: one cooperative launch runs every step (kernel3d_general). */
+enum perks_method { PERKS_METHOD_PLAIN = 0, PERKS_METHOD_PERSISTENT = 1 };
+
+#define PERKS_MAX_COEFS 27
+
+typedef struct perks_stencil_opts {
+    int32_t shape;          /* enum perks_shape */
+    int32_t boundary;       /* enum perks_boundary */
+    int32_t method;         /* enum perks_method */
+    int32_t use_ref_coefs;  /* 1: the shape's reference weights and divisor */
+    double coef[PERKS_MAX_COEFS]; /* per group, in the shape's summation order */
+    double divisor;         /* 0 = none (ignored when use_ref_coefs) */
+    /* tuning knobs; 0 = automatic.  Names follow j3d.driver.cpp:45-57. */
+    int32_t bdimx;          /* --bdim: 128 or 256 threads per workgroup */
+    int32_t blkpsm;         /* --blkpsm: cap on workgroups per CU */
+    int32_t use_lds_cache;  /* --usesm: 1 on, 0 auto, -1 off */
+    int32_t reg_cache_planes; /* VGPR-cached planes per workgroup; -1 auto */
+    int32_t double_tile;    /* --doubletile */
+    int32_t warmup;         /* --warmup: run untimed launches first */
+    int32_t warmup_steps;   /* --warmiter */
+    int32_t reserved[8];
+} perks_stencil_opts;
+
+typedef struct perks_run_report {
+    double elapsed_ms;     /* device time of the launch(es), hipEvents on `stream` */
+    double gcells_per_s;   /* steps * cells / elapsed (j3d.cu:754)               */
+    int32_t grid[3];
+    int32_t block;
+    int32_t lds_bytes;
+    int32_t lds_planes;    /* planes kept in LDS per workgroup (persistent)      */
+    int32_t reg_planes;    /* planes kept in VGPRs per workgroup (persistent)    */
+    int32_t result_in_b;   /* device entry points: 1 if the result is in d_b     */
+    int32_t occupancy;     /* workgroups per CU used                            */
+    int32_t reserved[7];
+} perks_run_report;
+
+/* Device facts the planner needs; perks_query_device fills it from HIP. */
+typedef struct perks_device_caps {
+    int32_t cu_count;
+    int32_t lds_per_cu;      /* bytes */
+    int32_t lds_per_block;   /* bytes */
+    int32_t vgprs_per_simd;  /* 32-bit registers per lane per SIMD */
+    int32_t simds_per_cu;
+    int32_t warp_size;
+    int32_t cooperative;
+    int32_t reserved[5];
+} perks_device_caps;
+
+typedef struct perks_plan_info {
+    int32_t grid[3];
+    int32_t block;
+    int32_t occupancy;       /* workgroups per CU                   */
+    int32_t lds_bytes;
+    int32_t lds_planes;
+    int32_t reg_planes;
+    int32_t min_height;      /* getminHeight result (j3d.cu:598)    */
+    int32_t tile_x, tile_y;
+    int32_t reserved[5];
+} perks_plan_info;
+
+/* ---- library ------------------------------------------------------------- */
+const char* perks_version(void);
+const char* perks_error_string(int code);
+int perks_query_device(int device, perks_device_caps* caps);
+/* Frees cached device scratch of the current device. */
+int perks_release(void);
+
+/* ---- shapes --------------------------------------------------------------- */
+/* Fills opts with defaults for `shape`: reference weights, CLAMP, PERSISTENT. */
+void perks_stencil_opts_init(perks_stencil_opts* opts, int shape);
+/* Describes a shape.  Any output pointer may be NULL.  coef/npts/prod_f64
+ * hold *ngroups entries; offsets holds ngroups*12*3 int8 (dz,dy,dx). */
+int perks_shape_info(int shape, int32_t* dims, int32_t* radius, int32_t* ngroups, double* coef,
+                     double* divisor, int32_t* npts, int32_t* prod_f64, int8_t* offsets);
+
+/* ---- planning (host only when caps != NULL) ------------------------------ */
+int perks_j3d_plan(int dtype, int height, int width_y, int width_x, const perks_stencil_opts* opts,
+                   const perks_device_caps* caps, int occupancy_hint, perks_plan_info* out);
+/* j3d_iterative(..., getminHeight=true): smallest height that caches fully. */
+int perks_j3d_min_height(int dtype, int width_y, int width_x, const perks_stencil_opts* opts);
+/* getMinWidthY<REAL>(width_x, width_y, bdimx, isDoubleTile), j3d.cu:69-123. */
+int perks_get_min_width_y(int dtype, int width_x, int width_y, int bdimx, int double_tile);
+
+/* ---- 3D Jacobi ------------------------------------------------------------ */
+/* Host buffers, as the reference: h_in[height][width_y][width_x] (x fastest)
+ * is read, h_out receives the field after `steps` steps.  The callee
+ * allocates and frees all device memory.  report may be NULL. */
+int perks_j3d_iterative_f64(const double* h_in, int height, int width_y, int width_x, double* h_out,
+                            int steps, const perks_stencil_opts* opts, perks_run_report* report);
+int perks_j3d_iterative_f32(const float* h_in, int height, int width_y, int width_x, float* h_out,
+                            int steps, const perks_stencil_opts* opts, perks_run_report* report);
+/* Device-resident buffers: d_a holds the input field, d_b is a second field
+ * of the same size.  Both are overwritten; after return (the work is
+ * enqueued on `stream` and the call waits for it), the result is in d_b
+ * when report->result_in_b is 1 (odd step counts), else in d_a. */
+int perks_j3d_device_f64(double* d_a, double* d_b, int height, int width_y, int width_x, int steps,
+                         const perks_stencil_opts* opts, void* stream, perks_run_report* report);
+int perks_j3d_device_f32(float* d_a, float* d_b, int height, int width_y, int width_x, int steps,
+                         const perks_stencil_opts* opts, void* stream, perks_run_report* report);
+
+/* ---- inputs ---------------------------------------------------------------- */
+/* d[i] = lo + (hi-lo) * u53(splitmix64(seed + (i+1)*0x9E3779B97F4A7C15)),
+ * the same sequence the host generator produces (float: rounded once). */
+int perks_fill_uniform_f64(double* d, size_t n, uint64_t seed, double lo, double hi, void* stream);
+int perks_fill_uniform_f32(float* d, size_t n, uint64_t seed, double lo, double hi, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PERKS_AMD_H */

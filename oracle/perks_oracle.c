@@ -1,0 +1,368 @@
+/*
+ * perks_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY; see perks_oracle.h).
+ *
+ * Build: oracle/Makefile, `-O2 -ffp-contract=off` with no FMA target flags so
+ * every product and sum is rounded exactly once, in the order written.
+ */
+#include "perks_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Inputs                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* SplitMix64 (Steele, Lea, Flood, "Fast splittable pseudorandom number
+ * generators", OOPSLA 2014; finaliser constants from Vigna's splitmix64.c),
+ * used counter-style: the state for element i is seed + (i+1)*golden, so any
+ * element can be generated independently (on the host or on the GPU). */
+uint64_t po_splitmix64(uint64_t seed, uint64_t index)
+{
+    uint64_t z = seed + (index + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double u53(uint64_t r) { return (double)(r >> 11) * 0x1.0p-53; }
+
+void po_fill_uniform_f64(double* out, size_t n, uint64_t seed, double lo, double hi)
+{
+    const double w = hi - lo;
+    for (size_t i = 0; i < n; ++i) out[i] = lo + w * u53(po_splitmix64(seed, i));
+}
+
+void po_fill_uniform_f32(float* out, size_t n, uint64_t seed, double lo, double hi)
+{
+    const double w = hi - lo;
+    for (size_t i = 0; i < n; ++i) out[i] = (float)(lo + w * u53(po_splitmix64(seed, i)));
+}
+
+/* stencil/3dstencil/common/common.hpp:24-27 (get_random) filled in z,y,x order
+ * (common.hpp:59-69). */
+void po_fill_libc_rand_f64(double* out, size_t n, unsigned seed)
+{
+    srand(seed);
+    for (size_t i = 0; i < n; ++i) out[i] = (double)rand() / (double)(RAND_MAX - 1);
+}
+
+void po_fill_libc_rand_f32(float* out, size_t n, unsigned seed)
+{
+    srand(seed);
+    for (size_t i = 0; i < n; ++i) out[i] = (float)rand() / (float)(RAND_MAX - 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stencil step                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Neighbour index along one axis.  Clamp = replicate the edge, as the gold's
+ * MIN/MAX index expressions do (3d7pt_gold_gen.cpp:21-26).  Zero = the cell
+ * lies in a halo that holds 0 (returns -1). */
+static inline int nb(int i, int d, int n, int bc)
+{
+    int j = i + d;
+    if (j < 0) return bc == PO_BC_CLAMP ? 0 : -1;
+    if (j >= n) return bc == PO_BC_CLAMP ? n - 1 : -1;
+    return j;
+}
+
+#define DEFINE_STEP(SUF, T)                                                                   \
+static void step_planes_##SUF(const T* in, T* out, int nz, int ny, int nx, const po_stencil* st, \
+                              int zb, int ze)                                                 \
+{                                                                                             \
+    const size_t plane = (size_t)ny * nx;                                                     \
+    const int bc = st->boundary;                                                              \
+    for (int z = zb; z < ze; ++z)                                                             \
+        for (int y = 0; y < ny; ++y)                                                          \
+            for (int x = 0; x < nx; ++x) {                                                    \
+                int in_dbl = 0;                                                               \
+                T acc_t = 0;                                                                  \
+                double acc_d = 0.0;                                                           \
+                for (int g = 0; g < st->ngroups; ++g) {                                       \
+                    const po_group* gr = &st->groups[g];                                      \
+                    T s = 0;                                                                  \
+                    for (int p = 0; p < gr->npts; ++p) {                                      \
+                        int zz = nb(z, gr->off[p][0], nz, bc);                                \
+                        int yy = nb(y, gr->off[p][1], ny, bc);                                \
+                        int xx = nb(x, gr->off[p][2], nx, bc);                                \
+                        T v = (zz < 0 || yy < 0 || xx < 0) ? (T)0                             \
+                                                           : in[zz * plane + (size_t)yy * nx + xx]; \
+                        s = (p == 0) ? v : (T)(s + v);                                        \
+                    }                                                                         \
+                    if (sizeof(T) == sizeof(double) || !gr->prod_f64) {                       \
+                        T prod = (T)((T)gr->coef * s);                                        \
+                        if (g == 0) acc_t = prod;                                             \
+                        else if (in_dbl) acc_d = acc_d + (double)prod;                        \
+                        else acc_t = (T)(acc_t + prod);                                       \
+                    } else {                                                                  \
+                        double prod = gr->coef * (double)s;                                   \
+                        if (g == 0) acc_d = prod;                                             \
+                        else if (in_dbl) acc_d = acc_d + prod;                                \
+                        else acc_d = (double)acc_t + prod;                                    \
+                        in_dbl = 1;                                                           \
+                    }                                                                         \
+                }                                                                             \
+                T r;                                                                          \
+                if (in_dbl) r = (T)(st->divisor != 0.0 ? acc_d / st->divisor : acc_d);        \
+                else r = (st->divisor != 0.0) ? (T)(acc_t / (T)st->divisor) : acc_t;          \
+                out[z * plane + (size_t)y * nx + x] = r;                                      \
+            }                                                                                 \
+}                                                                                             \
+                                                                                              \
+void po_stencil_step_##SUF(const T* in, T* out, int nz, int ny, int nx, const po_stencil* st) \
+{                                                                                             \
+    step_planes_##SUF(in, out, nz, ny, nx, st, 0, nz);                                        \
+}                                                                                             \
+                                                                                              \
+static void iterate_##SUF(const T* in, T* out, int nz, int ny, int nx, const po_stencil* st,  \
+                          int steps, int threads)                                             \
+{                                                                                             \
+    const size_t n = (size_t)nz * ny * nx;                                                    \
+    if (steps <= 0) { memcpy(out, in, n * sizeof(T)); return; }                               \
+    /* Ping-pong so that the last step writes `out` (j3d_gold_iterative's parity rule,      */\
+    /* 3d7pt_gold_gen.cpp:43-64): odd step counts start in `out`, even ones in `tmp`.       */\
+    T* tmp = (T*)malloc(n * sizeof(T));                                                       \
+    T* dst = (steps % 2 == 1) ? out : tmp;                                                    \
+    T* other = (steps % 2 == 1) ? tmp : out;                                                  \
+    const T* src = in;                                                                        \
+    for (int s = 0; s < steps; ++s) {                                                         \
+        if (threads > 1) {                                                                    \
+            _Pragma("omp parallel for num_threads(threads) schedule(static)")                \
+            for (int z = 0; z < nz; ++z) step_planes_##SUF(src, dst, nz, ny, nx, st, z, z + 1); \
+        } else {                                                                              \
+            step_planes_##SUF(src, dst, nz, ny, nx, st, 0, nz);                               \
+        }                                                                                     \
+        src = dst;                                                                            \
+        T* t = dst; dst = other; other = t;                                                   \
+    }                                                                                         \
+    free(tmp);                                                                                \
+}                                                                                             \
+void po_stencil_iterate_##SUF(const T* in, T* out, int nz, int ny, int nx, const po_stencil* st, int steps) \
+{ iterate_##SUF(in, out, nz, ny, nx, st, steps, 1); }                                         \
+void po_stencil_iterate_omp_##SUF(const T* in, T* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads) \
+{ iterate_##SUF(in, out, nz, ny, nx, st, steps, threads); }                                   \
+                                                                                              \
+double po_rms_error_##SUF(const T* a, const T* b, int ny, int nx, int z0, int z1, int y0, int y1, \
+                          int x0, int x1, double* max_err)                                    \
+{                                                                                             \
+    double err = 0.0, mx = 0.0;                                                               \
+    for (int z = z0; z < z1; ++z)                                                             \
+        for (int y = y0; y < y1; ++y)                                                         \
+            for (int x = x0; x < x1; ++x) {                                                   \
+                size_t i = ((size_t)z * ny + y) * nx + x;                                     \
+                double e = (double)a[i] - (double)b[i];                                       \
+                e = e < 0 ? -e : e;                                                           \
+                err += e * e;                                                                 \
+                if (e > mx) mx = e;                                                           \
+            }                                                                                 \
+    if (max_err) *max_err = mx;                                                               \
+    double cnt = (double)(z1 - z0) * (double)(y1 - y0) * (double)(x1 - x0);                   \
+    return cnt > 0 ? sqrt(err / cnt) : 0.0;                                                   \
+}
+
+DEFINE_STEP(f64, double)
+DEFINE_STEP(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* Sparse matrices                                                           */
+/* ------------------------------------------------------------------------ */
+
+int64_t po_poisson7_nnz(int nx, int ny, int nz)
+{
+    int64_t n = (int64_t)nx * ny * nz;
+    return 7 * n - 2 * ((int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny);
+}
+
+void po_poisson7_csr(int nx, int ny, int nz, int32_t* rowptr, int32_t* col, double* val)
+{
+    int64_t k = 0;
+    const int64_t sxy = (int64_t)nx * ny;
+    int64_t row = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x, ++row) {
+                rowptr[row] = (int32_t)k;
+                if (z > 0)      { col[k] = (int32_t)(row - sxy); val[k++] = -1.0; }
+                if (y > 0)      { col[k] = (int32_t)(row - nx);  val[k++] = -1.0; }
+                if (x > 0)      { col[k] = (int32_t)(row - 1);   val[k++] = -1.0; }
+                col[k] = (int32_t)row; val[k++] = 6.0;
+                if (x < nx - 1) { col[k] = (int32_t)(row + 1);   val[k++] = -1.0; }
+                if (y < ny - 1) { col[k] = (int32_t)(row + nx);  val[k++] = -1.0; }
+                if (z < nz - 1) { col[k] = (int32_t)(row + sxy); val[k++] = -1.0; }
+            }
+    rowptr[row] = (int32_t)k;
+}
+
+void po_poisson7_dia(int nx, int ny, int nz, int32_t* offsets, double* val)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int32_t sxy = nx * ny;
+    const int32_t offs[7] = {-sxy, -nx, -1, 0, 1, nx, sxy};
+    for (int d = 0; d < 7; ++d) offsets[d] = offs[d];
+    int64_t row = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x, ++row) {
+                val[0 * n + row] = z > 0 ? -1.0 : 0.0;
+                val[1 * n + row] = y > 0 ? -1.0 : 0.0;
+                val[2 * n + row] = x > 0 ? -1.0 : 0.0;
+                val[3 * n + row] = 6.0;
+                val[4 * n + row] = x < nx - 1 ? -1.0 : 0.0;
+                val[5 * n + row] = y < ny - 1 ? -1.0 : 0.0;
+                val[6 * n + row] = z < nz - 1 ? -1.0 : 0.0;
+            }
+}
+
+/* y_i = sum over stored entries of row i, in storage order, starting from 0. */
+void po_csr_spmv_f64(int n, const int32_t* rowptr, const int32_t* col, const double* val, const double* x, double* y)
+{
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * x[col[k]];
+        y[i] = s;
+    }
+}
+
+/* Diagonals in offset order; entries whose column falls outside [0,n) are skipped. */
+void po_dia_spmv_f64(int n, int ndiag, const int32_t* offsets, const double* val, const double* x, double* y)
+{
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int d = 0; d < ndiag; ++d) {
+            int64_t j = (int64_t)i + offsets[d];
+            if (j < 0 || j >= n) continue;
+            s = s + val[(size_t)d * n + i] * x[j];
+        }
+        y[i] = s;
+    }
+}
+
+/* Deterministic dot product.  The vector is cut into segments of 4096
+ * elements.  Inside a segment, lane l (0..255) sums elements l, l+256, ...,
+ * l+15*256 in that order starting from 0; the 256 lane sums are then folded
+ * pairwise with strides 128, 64, ..., 1 (v[l] += v[l+stride]).  The segment
+ * sums are combined the same way: lane l sums segments l, l+256, ... in order,
+ * then the same fold.  The GPU reduction is built to this exact order, so the
+ * FP64 CG residual history is reproducible bit for bit. */
+#define RED_LANES 256
+#define RED_PER_LANE 16
+#define RED_SEG (RED_LANES * RED_PER_LANE)
+
+static double fold256(double* v)
+{
+    for (int w = RED_LANES / 2; w >= 1; w >>= 1)
+        for (int l = 0; l < w; ++l) v[l] = v[l] + v[l + w];
+    return v[0];
+}
+
+static double reduce_partials(const double* p, size_t np)
+{
+    double v[RED_LANES];
+    for (int l = 0; l < RED_LANES; ++l) {
+        double acc = 0.0;
+        for (size_t j = (size_t)l; j < np; j += RED_LANES) acc = acc + p[j];
+        v[l] = acc;
+    }
+    return fold256(v);
+}
+
+double po_dot_f64(const double* a, const double* b, size_t n)
+{
+    size_t nseg = (n + RED_SEG - 1) / RED_SEG;
+    if (nseg == 0) return 0.0;
+    double* part = (double*)malloc(nseg * sizeof(double));
+    for (size_t s = 0; s < nseg; ++s) {
+        double v[RED_LANES];
+        for (int l = 0; l < RED_LANES; ++l) {
+            double acc = 0.0;
+            for (int k = 0; k < RED_PER_LANE; ++k) {
+                size_t i = s * RED_SEG + (size_t)k * RED_LANES + l;
+                if (i < n) acc = acc + a[i] * b[i];
+            }
+            v[l] = acc;
+        }
+        part[s] = fold256(v);
+    }
+    double r = reduce_partials(part, nseg);
+    free(part);
+    return r;
+}
+
+/* Same order; float operands, exact double products, double sums. */
+double po_dot_f32(const float* a, const float* b, size_t n)
+{
+    size_t nseg = (n + RED_SEG - 1) / RED_SEG;
+    if (nseg == 0) return 0.0;
+    double* part = (double*)malloc(nseg * sizeof(double));
+    for (size_t s = 0; s < nseg; ++s) {
+        double v[RED_LANES];
+        for (int l = 0; l < RED_LANES; ++l) {
+            double acc = 0.0;
+            for (int k = 0; k < RED_PER_LANE; ++k) {
+                size_t i = s * RED_SEG + (size_t)k * RED_LANES + l;
+                if (i < n) acc = acc + (double)a[i] * (double)b[i];
+            }
+            v[l] = acc;
+        }
+        part[s] = fold256(v);
+    }
+    double r = reduce_partials(part, nseg);
+    free(part);
+    return r;
+}
+
+static void spmv(int fmt, int n, const int32_t* rowptr, const int32_t* col, const double* val, int ndiag,
+                 const double* x, double* y)
+{
+    if (fmt == 0) po_csr_spmv_f64(n, rowptr, col, val, x, y);
+    else po_dia_spmv_f64(n, ndiag, col, val, x, y);
+}
+
+/* conjugateGradient/cg_kernels.cuh:721-975 (order of operations):
+ *   r = b + (-1)*A x ; r1 = r.r ; k = 1
+ *   loop while !(r1 < thr) and k <= max_iter:
+ *     k > 1 : beta = r1/r0 ; p = 1*r + beta*p      else p = r
+ *     Ap = A p ; alpha = r1 / (p.Ap)
+ *     x = alpha*p + x ; r = (-alpha)*Ap + r ; r0 = r1 ; r1 = r.r ; k++
+ *   iterations = k-1 */
+int po_cg_f64(int fmt, int n, const int32_t* rowptr, const int32_t* col, const double* val, int ndiag,
+              const double* b, double* x, double tol, int relative, int max_iter,
+              double* hist, int hist_cap)
+{
+    double* r = (double*)malloc((size_t)n * sizeof(double));
+    double* p = (double*)malloc((size_t)n * sizeof(double));
+    double* Ap = (double*)malloc((size_t)n * sizeof(double));
+    spmv(fmt, n, rowptr, col, val, ndiag, x, Ap);
+    for (int i = 0; i < n; ++i) r[i] = b[i] + (-1.0) * Ap[i];
+    double thr = tol * tol;
+    if (relative) thr = thr * po_dot_f64(b, b, (size_t)n);
+    double r1 = po_dot_f64(r, r, (size_t)n), r0 = 0.0;
+    if (hist && hist_cap > 0) hist[0] = r1;
+    int k = 1;
+    while (!(r1 < thr) && k <= max_iter) {
+        if (k > 1) {
+            double beta = r1 / r0;
+            for (int i = 0; i < n; ++i) p[i] = 1.0 * r[i] + beta * p[i];
+        } else {
+            memcpy(p, r, (size_t)n * sizeof(double));
+        }
+        spmv(fmt, n, rowptr, col, val, ndiag, p, Ap);
+        double pAp = po_dot_f64(p, Ap, (size_t)n);
+        double alpha = r1 / pAp;
+        double na = -alpha;
+        for (int i = 0; i < n; ++i) x[i] = alpha * p[i] + x[i];
+        for (int i = 0; i < n; ++i) r[i] = na * Ap[i] + r[i];
+        r0 = r1;
+        r1 = po_dot_f64(r, r, (size_t)n);
+        if (hist && k < hist_cap) hist[k] = r1;
+        ++k;
+    }
+    free(r); free(p); free(Ap);
+    return k - 1;
+}

@@ -1,0 +1,108 @@
+/*
+ * perks_oracle.h -- CPU oracle for the Jacobi stencil and CG paths.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product library (libperks_amd.so) never
+ * links or calls it.
+ *
+ * What it restates (reference paths relative to the PERKS source tree):
+ *   - stencil/3dstencil/3d7pt/3d7pt_gold_gen.cpp:9-36   j3d_step (clamp, every cell)
+ *   - stencil/3dstencil/3d7pt/3d7pt_gold_gen.cpp:38-67  j3d_gold_iterative (ping-pong parity)
+ *   - stencil/3dstencil/3d7pt/3d7pt_gold.cpp:8-25       interior-only step (Dirichlet form)
+ *   - stencil/3dstencil/{3d13pt,3d17pt,3d27pt,poisson}/ *_gold_gen.cpp   other 3D shapes
+ *   - stencil/2dstencil/2d*pt/jacobi_gold_gen.cpp       2D shapes (a 2D grid is nz == 1)
+ *   - stencil/3dstencil/common/common.hpp:24-27,105-163 rand()-based input and error metrics
+ *   - conjugateGradient/cg_kernels.cuh:721-975          CG algorithm order (here in FP64,
+ *     with the reduction order fixed as documented at po_dot_f64)
+ *
+ * A stencil is an ordered list of "groups".  Group g contributes
+ *     coef_g * (x[p_0] + x[p_1] + ... + x[p_{n-1}])          (points summed left to right)
+ * and the groups are accumulated left to right starting from the first group's
+ * product, exactly as a C expression  c0*a + c1*b + ... would be evaluated.
+ * An optional final division by `divisor` follows.  For float storage a group
+ * flagged prod_f64 forms its product in double (the reference writes such
+ * coefficients as double literals, e.g. `15.0*input`), and from the first such
+ * group on the running sum is held in double; the division is then in double
+ * too and the result is rounded to float on store.  For double storage every
+ * flag is irrelevant.
+ */
+#ifndef PERKS_ORACLE_H
+#define PERKS_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PO_MAX_GROUPS 64
+#define PO_MAX_POINTS 12
+
+enum { PO_BC_CLAMP = 0, PO_BC_ZERO = 1 };
+
+typedef struct po_group {
+    double coef;
+    int32_t npts;
+    int32_t prod_f64;
+    int32_t off[PO_MAX_POINTS][3]; /* (dz, dy, dx) */
+} po_group;
+
+typedef struct po_stencil {
+    int32_t ngroups;
+    int32_t boundary; /* PO_BC_CLAMP or PO_BC_ZERO */
+    double divisor;   /* 0 => no division */
+    po_group groups[PO_MAX_GROUPS];
+} po_stencil;
+
+/* ---- inputs ---------------------------------------------------------- */
+uint64_t po_splitmix64(uint64_t seed, uint64_t index);
+void po_fill_uniform_f64(double* out, size_t n, uint64_t seed, double lo, double hi);
+void po_fill_uniform_f32(float* out, size_t n, uint64_t seed, double lo, double hi);
+/* the reference's generator: (T)rand()/(T)(RAND_MAX-1) after srand(seed) */
+void po_fill_libc_rand_f64(double* out, size_t n, unsigned seed);
+void po_fill_libc_rand_f32(float* out, size_t n, unsigned seed);
+
+/* ---- stencil --------------------------------------------------------- */
+void po_stencil_step_f64(const double* in, double* out, int nz, int ny, int nx, const po_stencil* st);
+void po_stencil_step_f32(const float* in, float* out, int nz, int ny, int nx, const po_stencil* st);
+/* Runs `steps` steps.  The result always lands in `out` (as the reference's
+ * j3d_gold_iterative guarantees); `in` is not modified.  steps == 0 copies. */
+void po_stencil_iterate_f64(const double* in, double* out, int nz, int ny, int nx, const po_stencil* st, int steps);
+void po_stencil_iterate_f32(const float* in, float* out, int nz, int ny, int nx, const po_stencil* st, int steps);
+/* Same, with the z loop split across `threads` OpenMP threads (bitwise equal). */
+void po_stencil_iterate_omp_f64(const double* in, double* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
+void po_stencil_iterate_omp_f32(const float* in, float* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
+
+/* RMS and max |a-b| over a box [z0,z1)x[y0,y1)x[x0,x1) (common.hpp:133-163). */
+double po_rms_error_f64(const double* a, const double* b, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1, double* max_err);
+double po_rms_error_f32(const float* a, const float* b, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1, double* max_err);
+
+/* ---- sparse matrices and CG ----------------------------------------- */
+/* 3D Poisson 7-point matrix on an n0 x n1 x n2 grid (x fastest), Dirichlet:
+ * diagonal 6, off-diagonals -1, columns ascending.  rowptr has N+1 entries. */
+int64_t po_poisson7_nnz(int nx, int ny, int nz);
+void po_poisson7_csr(int nx, int ny, int nz, int32_t* rowptr, int32_t* col, double* val);
+/* DIA: 7 diagonals with offsets {-nx*ny, -nx, -1, 0, 1, nx, nx*ny}, stored
+ * val[d*N + i] = A[i][i+off_d] (0 where i+off_d falls outside or crosses a face). */
+void po_poisson7_dia(int nx, int ny, int nz, int32_t* offsets, double* val);
+
+void po_csr_spmv_f64(int n, const int32_t* rowptr, const int32_t* col, const double* val, const double* x, double* y);
+void po_dia_spmv_f64(int n, int ndiag, const int32_t* offsets, const double* val, const double* x, double* y);
+double po_dot_f64(const double* a, const double* b, size_t n);
+double po_dot_f32(const float* a, const float* b, size_t n);
+
+/* FP64 CG following conjugateGradient/cg_kernels.cuh:721-975.  fmt 0 = CSR
+ * (rowptr/col/val), 1 = DIA (offsets in `col`, ndiag in `ndiag`).
+ * Stops when r.r < tol^2 * (relative ? b.b : 1) or after max_iter iterations.
+ * hist[k] = r.r after iteration k (hist[0] = initial), up to hist_cap entries.
+ * Returns the iteration count. */
+int po_cg_f64(int fmt, int n, const int32_t* rowptr, const int32_t* col, const double* val, int ndiag,
+              const double* b, double* x, double tol, int relative, int max_iter,
+              double* hist, int hist_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

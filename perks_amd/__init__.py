@@ -1,0 +1,15 @@
+"""perks_amd -- persistent Jacobi stencils and CG for MI355X (gfx950).
+
+The compute lives in libperks_amd.so (hand-written HIP kernels behind a C ABI,
+include/perks_amd.h).  This package is the host-side mirror of the reference's
+entry points; it loads the library and fails loudly if it is missing.
+"""
+from . import _lib
+from ._lib import PerksError, check, load
+from .stencil import (fill_uniform, j3d_device, j3d_iterative, j3d_run, make_opts, plan, query_device, shape_id,
+                      shape_info)
+
+__all__ = [
+    "PerksError", "check", "load", "fill_uniform", "j3d_device", "j3d_iterative", "j3d_run", "make_opts", "plan",
+    "query_device", "shape_id", "shape_info", "_lib",
+]

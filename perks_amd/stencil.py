@@ -1,0 +1,191 @@
+"""Host-side mirror of the reference's Jacobi entry points over the C ABI.
+
+``j3d_iterative`` keeps the argument names, meaning and return codes of the
+reference's ``template<class REAL> int j3d_iterative(REAL* h_input, int height,
+int width_y, int width_x, REAL* h_output, int bdimx, int blkpsm, int iteration,
+bool useSM, bool usewarmup, int warmupiteration, bool isDoubleTile, bool
+getminHeight=false)`` (stencil/3dstencil/common/jacobi_cuda.cuh:7-8,
+stencil/3dstencil/j3d.cu:129-140): host arrays in and out, 0 on success, the
+planner's negative codes on refusal.  The stencil shape, boundary rule, method
+and coefficients, which the reference fixes at compile time, are keyword
+arguments here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+BOUNDARIES = {"clamp": L.BC_CLAMP, "zero": L.BC_ZERO}
+METHODS = {"plain": L.METHOD_PLAIN, "persistent": L.METHOD_PERSISTENT}
+
+
+@dataclass
+class ShapeInfo:
+    name: str
+    id: int
+    dims: int
+    radius: int
+    coef: list
+    divisor: float
+    npts: list
+    prod_f64: list
+    offsets: list  # per group: list of (dz, dy, dx)
+
+
+def shape_id(shape) -> int:
+    if isinstance(shape, int):
+        return shape
+    if shape in L.SHAPES:
+        return L.SHAPES[shape]
+    if shape in L.REF_DIR_SHAPE:
+        return L.SHAPES[L.REF_DIR_SHAPE[shape]]
+    raise ValueError(f"unknown shape {shape!r}")
+
+
+def shape_info(shape) -> ShapeInfo:
+    lib = L.load()
+    sid = shape_id(shape)
+    dims, radius, ng = C.c_int32(), C.c_int32(), C.c_int32()
+    coef = (C.c_double * L.MAX_COEFS)()
+    div = C.c_double()
+    npts = (C.c_int32 * L.MAX_COEFS)()
+    pf = (C.c_int32 * L.MAX_COEFS)()
+    offs = (C.c_int8 * (L.MAX_COEFS * L.MAX_PTS * 3))()
+    L.check(lib.perks_shape_info(sid, C.byref(dims), C.byref(radius), C.byref(ng), coef, C.byref(div), npts, pf,
+                                 offs), "perks_shape_info")
+    n = ng.value
+    offsets = []
+    for g in range(n):
+        pts = []
+        for p in range(npts[g]):
+            b = (g * L.MAX_PTS + p) * 3
+            pts.append((offs[b], offs[b + 1], offs[b + 2]))
+        offsets.append(pts)
+    name = next((k for k, v in L.SHAPES.items() if v == sid), str(sid))
+    return ShapeInfo(name, sid, dims.value, radius.value, list(coef[:n]), div.value, list(npts[:n]), list(pf[:n]),
+                     offsets)
+
+
+def make_opts(shape="3d_star_r1", boundary="clamp", method="persistent", coefs=None, divisor=None,
+              bdimx=0, blkpsm=0, use_lds_cache=0, reg_cache_planes=-1, double_tile=0, warmup=0,
+              warmup_steps=0, reserved=None) -> L.StencilOpts:
+    lib = L.load()
+    o = L.StencilOpts()
+    lib.perks_stencil_opts_init(C.byref(o), shape_id(shape))
+    o.boundary = BOUNDARIES[boundary] if isinstance(boundary, str) else int(boundary)
+    o.method = METHODS[method] if isinstance(method, str) else int(method)
+    if coefs is not None:
+        o.use_ref_coefs = 0
+        for i, c in enumerate(coefs):
+            o.coef[i] = float(c)
+        o.divisor = 0.0 if divisor is None else float(divisor)
+    o.bdimx, o.blkpsm, o.use_lds_cache = int(bdimx), int(blkpsm), int(use_lds_cache)
+    o.reg_cache_planes, o.double_tile = int(reg_cache_planes), int(double_tile)
+    o.warmup, o.warmup_steps = int(warmup), int(warmup_steps)
+    if reserved:
+        for i, v in enumerate(reserved):
+            o.reserved[i] = int(v)
+    return o
+
+
+def _dtype_fn(arr: np.ndarray, kind: str):
+    lib = L.load()
+    if arr.dtype == np.float64:
+        return getattr(lib, f"perks_{kind}_f64")
+    if arr.dtype == np.float32:
+        return getattr(lib, f"perks_{kind}_f32")
+    raise TypeError(f"unsupported dtype {arr.dtype}")
+
+
+def j3d_iterative(h_input, height, width_y, width_x, h_output, bdimx=256, blkpsm=0, iteration=3, useSM=False,
+                  usewarmup=False, warmupiteration=-1, isDoubleTile=False, getminHeight=False, *,
+                  shape="3d_star_r1", boundary="clamp", method="persistent", coefs=None, divisor=None,
+                  report: L.RunReport | None = None, **tuning) -> int:
+    """Runs `iteration` Jacobi steps of h_input[height][width_y][width_x] into
+    h_output.  Returns 0, a negative planner code, or (getminHeight) the
+    minimum height that caches fully -- as the reference does."""
+    opts = make_opts(shape, boundary, method, coefs, divisor, bdimx=bdimx, blkpsm=blkpsm,
+                     use_lds_cache=1 if useSM else -1, double_tile=int(bool(isDoubleTile)),
+                     warmup=int(bool(usewarmup)), warmup_steps=max(0, int(warmupiteration)), **tuning)
+    lib = L.load()
+    if getminHeight:
+        dt = L.F64 if np.dtype(h_input.dtype if h_input is not None else np.float64) == np.float64 else L.F32
+        return lib.perks_j3d_min_height(dt, width_y, width_x, C.byref(opts))
+    for a in (h_input, h_output):
+        if not (isinstance(a, np.ndarray) and a.flags.c_contiguous and a.size == height * width_y * width_x):
+            raise ValueError("h_input/h_output must be C-contiguous arrays of height*width_y*width_x elements")
+    if h_input.dtype != h_output.dtype:
+        raise TypeError("h_input and h_output must share a dtype")
+    rep = report if report is not None else L.RunReport()
+    fn = _dtype_fn(h_input, "j3d_iterative")
+    return fn(h_input.ctypes.data, height, width_y, width_x, h_output.ctypes.data, int(iteration), C.byref(opts),
+              C.byref(rep))
+
+
+def j3d_run(field: np.ndarray, steps: int, **kw) -> tuple[np.ndarray, L.RunReport]:
+    """Convenience form: returns (result, report); raises on any error code."""
+    field = np.ascontiguousarray(field)
+    nz, ny, nx = field.shape
+    out = np.empty_like(field)
+    rep = L.RunReport()
+    rc = j3d_iterative(field, nz, ny, nx, out, iteration=steps, report=rep, **kw)
+    L.check(rc, "j3d_iterative")
+    return out, rep
+
+
+def _ptr(t) -> int:
+    if isinstance(t, int):
+        return t
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    raise TypeError("expected a device pointer (int) or a torch tensor")
+
+
+def j3d_device(d_a, d_b, height, width_y, width_x, steps, dtype="f64", stream=None, opts=None,
+               **kw) -> L.RunReport:
+    """Runs on device-resident buffers (torch tensors or raw pointers).  The
+    result is in d_b when report.result_in_b, else in d_a."""
+    lib = L.load()
+    o = opts if opts is not None else make_opts(**kw)
+    rep = L.RunReport()
+    fn = lib.perks_j3d_device_f64 if dtype in ("f64", L.F64) else lib.perks_j3d_device_f32
+    s = None if stream is None else _ptr_stream(stream)
+    L.check(fn(_ptr(d_a), _ptr(d_b), height, width_y, width_x, int(steps), C.byref(o), s, C.byref(rep)),
+            "j3d_device")
+    return rep
+
+
+def _ptr_stream(stream) -> int:
+    if isinstance(stream, int):
+        return stream
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    raise TypeError("expected a hipStream_t (int) or a torch stream")
+
+
+def fill_uniform(d, n, seed, lo, hi, dtype="f64", stream=None) -> None:
+    lib = L.load()
+    fn = lib.perks_fill_uniform_f64 if dtype in ("f64", L.F64) else lib.perks_fill_uniform_f32
+    s = None if stream is None else _ptr_stream(stream)
+    L.check(fn(_ptr(d), int(n), int(seed), float(lo), float(hi), s), "fill_uniform")
+
+
+def query_device(device: int = 0) -> L.DeviceCaps:
+    caps = L.DeviceCaps()
+    L.check(L.load().perks_query_device(device, C.byref(caps)), "query_device")
+    return caps
+
+
+def plan(height, width_y, width_x, dtype="f64", caps: L.DeviceCaps | None = None, occupancy=0,
+         **kw) -> tuple[int, L.PlanInfo]:
+    """Pure host planning when `caps` is given.  Returns (code, plan)."""
+    o = make_opts(**kw)
+    info = L.PlanInfo()
+    dt = L.F64 if dtype in ("f64", L.F64) else L.F32
+    rc = L.load().perks_j3d_plan(dt, height, width_y, width_x, C.byref(o), C.byref(caps) if caps else None,
+                                 int(occupancy), C.byref(info))
+    return rc, info

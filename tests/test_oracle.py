@@ -1,0 +1,203 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+* against the reference's own CPU gold, compiled from its sources into
+  oracle/_ref (every stencil shape, FP64 and FP32, the reference's rand() input);
+* against the reference's interior-only 4-step j3d_gold for the zero-halo form;
+* against closed forms (a separable sine mode is an eigenvector of the
+  symmetric star stencils with zero halo; a constant field under clamp);
+* against an independent numpy restatement (padding instead of index clamps)
+  on small grids, bit for bit;
+* the counter RNG against SplitMix64's published first outputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import perks_amd as P
+
+REF_3D = {"3d7pt": "3d7pt_gold_gen", "3d13pt": "3d13pt_gold_gen", "3d17pt": "3d17pt_gold_gen",
+          "3d27pt": "3d27pt_gold_gen", "poisson": "poisson_gold_gen"}
+REF_2D = {"2d5pt": "2d5pt_gold_gen", "2ds9pt": "2ds9pt_gold_gen", "2d13pt": "2d13pt_gold_gen",
+          "2d17pt": "2d17pt_gold_gen", "2d21pt": "2d21pt_gold_gen", "2ds25pt": "2ds25pt_gold_gen",
+          "2d9pt": "2d9pt_gold_gen", "2d25pt": "2d25pt_gold_gen"}
+
+
+def need_ref(name):
+    if not O.ref_available(name):
+        pytest.skip(f"oracle/_ref/libref_{name}.so not built (reference sources absent)")
+
+
+# ---------------------------------------------------------------- RNG
+def test_splitmix64_known_answers():
+    # SplitMix64 from state 0: published first outputs (Vigna's splitmix64.c).
+    assert O.lib().po_splitmix64(0, 0) == 0xE220A8397B1DCDAF
+    assert O.lib().po_splitmix64(0, 1) == 0x6E789E6AA1B965F4
+    assert O.lib().po_splitmix64(0, 2) == 0x06C45D188009454F
+
+
+def test_uniform_range_and_determinism():
+    a = O.uniform(100000, 7, 0.0, 1.0)
+    b = O.uniform(100000, 7, 0.0, 1.0)
+    assert np.array_equal(a, b)
+    assert a.min() >= 0.0 and a.max() < 1.0
+    assert abs(a.mean() - 0.5) < 0.01
+    c = O.uniform(1000, 8, -1.0, 1.0)
+    assert c.min() >= -1.0 and c.max() < 1.0
+    assert not np.array_equal(a[:1000], O.uniform(1000, 8, 0.0, 1.0))
+
+
+# ---------------------------------------------------------------- reference gold
+@pytest.mark.parametrize("ref", sorted(REF_3D))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dims,steps", [((5, 6, 7), 1), ((9, 8, 12), 4), ((4, 5, 3), 3)])
+def test_oracle_matches_reference_gold_3d(ref, dtype, dims, steps):
+    need_ref(REF_3D[ref])
+    si = P.shape_info(ref)
+    field = O.libc_rand(int(np.prod(dims)), 1, dtype).reshape(dims)
+    want = O.ref_gold_iterative(REF_3D[ref], field, steps)
+    got = O.iterate(field, O.stencil_from_shape(si, "clamp"), steps)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), ref
+
+
+@pytest.mark.parametrize("ref", sorted(REF_2D))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dims,steps", [((13, 17), 1), ((20, 16), 5), ((7, 9), 2)])
+def test_oracle_matches_reference_gold_2d(ref, dtype, dims, steps):
+    need_ref(REF_2D[ref])
+    si = P.shape_info(ref)
+    field = O.libc_rand(int(np.prod(dims)), 3, dtype).reshape(dims)
+    want = O.ref_gold_iterative(REF_2D[ref], field, steps)
+    got = O.iterate(field, O.stencil_from_shape(si, "clamp"), steps)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), ref
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_zero_halo_matches_reference_interior_gold(dtype):
+    """3d7pt_gold.cpp's j3d_gold updates only interior cells and, after its
+    first step, sees a zero boundary; with a zero boundary in the input it is
+    four zero-halo steps on the interior."""
+    need_ref("3d7pt_gold")
+    n = (10, 9, 11)
+    field = O.libc_rand(int(np.prod(n)), 5, dtype).reshape(n)
+    field[0, :, :] = field[-1, :, :] = 0
+    field[:, 0, :] = field[:, -1, :] = 0
+    field[:, :, 0] = field[:, :, -1] = 0
+    want = O.ref_j3d_gold_4step(field)
+    si = P.shape_info("3d7pt")
+    got = O.iterate(np.ascontiguousarray(field[1:-1, 1:-1, 1:-1]), O.stencil_from_shape(si, "zero"), 4)
+    assert np.array_equal(got, want[1:-1, 1:-1, 1:-1])
+
+
+# ---------------------------------------------------------------- independent numpy restatement
+def numpy_step(u, groups, boundary, divisor=0.0):
+    """Pads by the radius (edge replicate or zeros) and slices; same term order."""
+    r = max(max(abs(c) for pt in pts for c in pt) for _, pts, _ in groups)
+    mode = "edge" if boundary == "clamp" else "constant"
+    up = np.pad(u, r, mode=mode)
+    nz, ny, nx = u.shape
+
+    def view(dz, dy, dx):
+        return up[r + dz:r + dz + nz, r + dy:r + dy + ny, r + dx:r + dx + nx]
+
+    acc = None
+    for coef, pts, _ in groups:
+        s = view(*pts[0])
+        for pt in pts[1:]:
+            s = s + view(*pt)
+        prod = u.dtype.type(coef) * s
+        acc = prod if acc is None else acc + prod
+    if divisor:
+        acc = acc / u.dtype.type(divisor)
+    return acc
+
+
+CONFIG_3D = [0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4]  # E W N S T B C: neighbours 0.1, centre 0.4
+
+
+@pytest.mark.parametrize("boundary", ["clamp", "zero"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_oracle_matches_numpy_restatement(boundary, dtype):
+    si = P.shape_info("3d_star_r1")
+    groups = [(CONFIG_3D[g], si.offsets[g], 0) for g in range(7)]
+    u = O.uniform(6 * 7 * 9, 11, 0.0, 1.0, dtype).reshape(6, 7, 9)
+    want = u
+    for _ in range(3):
+        want = numpy_step(want, groups, boundary)
+    got = O.iterate(u, O.stencil(groups, boundary), 3)
+    assert np.array_equal(got, want)
+
+
+def test_oracle_box_matches_numpy_restatement_f64():
+    si = P.shape_info("3d_box_r1")
+    groups = [(si.coef[g], si.offsets[g], 1) for g in range(len(si.coef))]
+    u = O.uniform(5 * 6 * 7, 12, -1.0, 1.0).reshape(5, 6, 7)
+    want = numpy_step(numpy_step(u, groups, "zero", si.divisor), groups, "zero", si.divisor)
+    got = O.iterate(u, O.stencil(groups, "zero", si.divisor), 2)
+    assert np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------- closed forms
+@pytest.mark.parametrize("n", [7, 16])
+def test_sine_mode_is_eigenvector_3d(n):
+    th = math.pi / (n + 1)
+    s = np.sin(np.arange(1, n + 1) * th)
+    u0 = np.einsum("k,j,i->kji", s, s, s).copy()
+    lam = 0.4 + 0.6 * math.cos(th)
+    steps = 10
+    got = O.iterate(u0, O.stencil([(c, [o], 0) for c, o in zip(CONFIG_3D, [
+        (0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0), (0, 0, 0)])], "zero"), steps)
+    np.testing.assert_allclose(got, lam ** steps * u0, rtol=0, atol=1e-13)
+
+
+def test_sine_mode_is_eigenvector_2d():
+    n = 31
+    th = math.pi / (n + 1)
+    s = np.sin(np.arange(1, n + 1) * th)
+    u0 = np.outer(s, s)
+    si = P.shape_info("2d5pt")
+    st = O.stencil_from_shape(si, "zero", coefs=[0.2] * 5)
+    lam = 0.2 + 0.8 * math.cos(th)
+    got = O.iterate(u0, st, 25)
+    np.testing.assert_allclose(got, lam ** 25 * u0, rtol=0, atol=1e-13)
+
+
+def test_constant_field_under_clamp():
+    si = P.shape_info("3d7pt")
+    c = 0.75
+    u = np.full((4, 5, 6), c)
+    got = O.iterate(u, O.stencil_from_shape(si, "clamp"), 3)
+    ssum = sum(si.coef)
+    np.testing.assert_allclose(got, c * ssum ** 3, rtol=1e-14)
+
+
+def test_zero_steps_copies_and_parity():
+    si = P.shape_info("3d7pt")
+    u = O.uniform(3 * 4 * 5, 2, 0, 1).reshape(3, 4, 5)
+    assert np.array_equal(O.iterate(u, O.stencil_from_shape(si), 0), u)
+    one = O.iterate(u, O.stencil_from_shape(si), 1)
+    two = O.iterate(one, O.stencil_from_shape(si), 1)
+    assert np.array_equal(O.iterate(u, O.stencil_from_shape(si), 2), two)
+
+
+def test_zero_field_stays_zero():
+    si = P.shape_info("3d_box_r1")
+    u = np.zeros((4, 4, 4))
+    assert not O.iterate(u, O.stencil_from_shape(si, "zero"), 3).any()
+
+
+def test_omp_is_bitwise_equal():
+    si = P.shape_info("3d7pt")
+    u = O.uniform(24 * 20 * 33, 4, 0, 1).reshape(24, 20, 33)
+    st = O.stencil_from_shape(si, "zero", coefs=CONFIG_3D)
+    assert np.array_equal(O.iterate(u, st, 3), O.iterate(u, st, 3, threads=4))
+
+
+def test_rms_error_metric():
+    a = np.zeros((2, 3, 4))
+    b = np.zeros((2, 3, 4))
+    b[1, 2, 3] = 3.0
+    rms, mx = O.rms_error(a, b)
+    assert mx == 3.0
+    assert abs(rms - math.sqrt(9.0 / 24)) < 1e-15
