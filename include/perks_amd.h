@@ -94,7 +94,11 @@ typedef struct perks_stencil_opts {
     int32_t double_tile;    /* --doubletile */
     int32_t warmup;         /* --warmup: run untimed launches first */
     int32_t warmup_steps;   /* --warmiter */
-    int32_t reserved[8];
+    /* experimental knobs, 0 = automatic:
+     *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 1 = nontemporal stores
+     *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
+     *   tune[4..7] unused */
+    int32_t tune[8];
 } perks_stencil_opts;
 
 typedef struct perks_run_report {

@@ -50,7 +50,7 @@ class StencilOpts(C.Structure):
         ("coef", C.c_double * MAX_COEFS), ("divisor", C.c_double),
         ("bdimx", C.c_int32), ("blkpsm", C.c_int32), ("use_lds_cache", C.c_int32),
         ("reg_cache_planes", C.c_int32), ("double_tile", C.c_int32), ("warmup", C.c_int32),
-        ("warmup_steps", C.c_int32), ("reserved", C.c_int32 * 8),
+        ("warmup_steps", C.c_int32), ("tune", C.c_int32 * 8),
     ]
 
 

@@ -15,6 +15,7 @@
 #include "api_common.h"
 #include "j3d_kernels.h"
 #include "j3d_persistent.h"
+#include "planner.h"
 
 namespace perks {
 
@@ -38,13 +39,18 @@ int device_caps_cached(perks_device_caps* caps)
 namespace {
 
 template <typename T>
-int run_plain(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf, int bc, hipStream_t stream,
-              perks_run_report* rep)
+int run_plain(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf, const perks_stencil_opts* o,
+              hipStream_t stream, perks_run_report* rep)
 {
     perks_device_caps caps;
     int rc = device_caps_cached(&caps);
     if (rc != PERKS_OK) return rc;
-    const int zchunk = plain_default_zchunk(nz, ny, nx, caps.cu_count, 8);
+    perks_plan_info pl;
+    rc = plan_plain(nz, ny, nx, caps.cu_count, &pl);
+    if (rc != PERKS_OK) return rc;
+    const int zchunk = o->tune[0] > 0 ? o->tune[0] : pl.reserved[0];
+    const int variant = o->tune[1] ? 1 : 0;
+    const int bc = o->boundary;
     hipEvent_t ev0, ev1;
     PERKS_HIP_TRY(hipEventCreate(&ev0));
     PERKS_HIP_TRY(hipEventCreate(&ev1));
@@ -53,7 +59,7 @@ int run_plain(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& c
     T* src = d_a;
     T* dst = d_b;
     for (int s = 0; s < steps && rc == PERKS_OK; ++s) {
-        rc = launch_j3d_star1_plain<T>(src, dst, nz, ny, nx, cf, bc, zchunk, stream, &geom);
+        rc = launch_j3d_star1_plain<T>(src, dst, nz, ny, nx, cf, bc, zchunk, stream, &geom, variant);
         T* t = src;
         src = dst;
         dst = t;
@@ -106,7 +112,7 @@ int j3d_device(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const perks_st
     if (rc != PERKS_OK) return rc;
     if (rep) std::memset(rep, 0, sizeof(*rep));
     hipStream_t s = (hipStream_t)stream;
-    if (o->method == PERKS_METHOD_PLAIN) return run_plain<T>(d_a, d_b, nz, ny, nx, steps, cf, o->boundary, s, rep);
+    if (o->method == PERKS_METHOD_PLAIN) return run_plain<T>(d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
     return run_persistent<T>(d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
 }
 

@@ -13,11 +13,11 @@ struct PlainGeom {
     int zchunk;
 };
 
-int plain_default_zchunk(int nz, int ny, int nx, int cu_count, int ny_per_thread);
 
 // One time step of the 3D star r1 (7-point) stencil: out = S(in).
+// variant bit 0: nontemporal stores.
 template <typename T>
 int launch_j3d_star1_plain(const T* in, T* out, int nz, int ny, int nx, const KCoefs& cf, int bc, int zchunk,
-                           hipStream_t stream, PlainGeom* geom);
+                           hipStream_t stream, PlainGeom* geom, int variant);
 
 }  // namespace perks

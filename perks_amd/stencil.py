@@ -72,7 +72,7 @@ def shape_info(shape) -> ShapeInfo:
 
 def make_opts(shape="3d_star_r1", boundary="clamp", method="persistent", coefs=None, divisor=None,
               bdimx=0, blkpsm=0, use_lds_cache=0, reg_cache_planes=-1, double_tile=0, warmup=0,
-              warmup_steps=0, reserved=None) -> L.StencilOpts:
+              warmup_steps=0, tune=None) -> L.StencilOpts:
     lib = L.load()
     o = L.StencilOpts()
     lib.perks_stencil_opts_init(C.byref(o), shape_id(shape))
@@ -86,9 +86,9 @@ def make_opts(shape="3d_star_r1", boundary="clamp", method="persistent", coefs=N
     o.bdimx, o.blkpsm, o.use_lds_cache = int(bdimx), int(blkpsm), int(use_lds_cache)
     o.reg_cache_planes, o.double_tile = int(reg_cache_planes), int(double_tile)
     o.warmup, o.warmup_steps = int(warmup), int(warmup_steps)
-    if reserved:
-        for i, v in enumerate(reserved):
-            o.reserved[i] = int(v)
+    if tune:
+        for i, v in enumerate(tune):
+            o.tune[i] = int(v)
     return o
 
 
