@@ -97,7 +97,7 @@ typedef struct perks_stencil_opts {
     /* experimental knobs, 0 = automatic:
      *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 1 = nontemporal stores
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
-     *   tune[4..7] unused */
+     *   tune[4] persistent: tile variant (0..3)      tune[5..7] unused */
     int32_t tune[8];
 } perks_stencil_opts;
 

@@ -29,6 +29,24 @@ __device__ __forceinline__ double dpp_move(double src, double old)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Whole-wave shift by one lane with zero fill (bound_ctrl): one DPP move per
+// 32-bit half and no `old` operand to set up.  Used where the end lanes are
+// ghost lanes whose values are never consumed.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift0(float src)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(src), CTRL, 0xF, 0xF, true));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift0(double src)
+{
+    const long long s = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_mov_dpp((int)s, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(s >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Value of `v` in lane `lane` (wave-uniform result).
 __device__ __forceinline__ float read_lane(float v, int lane)
 {

@@ -90,3 +90,45 @@ def test_device_api_parity_and_result_buffer(gpu):
         want = O.iterate(O.uniform(n, 9, 0.0, 1.0).reshape(dims), O.stencil_from_shape(si, "zero", coefs=CONFIG_3D),
                          steps)
         assert np.array_equal(res, want)
+
+
+PERSIST_CASES = [
+    ((16, 64, 128), 1, 0),
+    ((16, 64, 128), 2, 0),
+    ((33, 40, 70), 5, 0),      # ragged in every axis, steps not a multiple of the pass depth
+    ((5, 7, 9), 4, 0),         # smaller than one tile
+    ((64, 96, 192), 7, 0),
+    ((1, 33, 65), 3, 0),       # single plane
+    ((40, 70, 130), 6, 1),     # one step per pass
+    ((40, 70, 130), 6, 2),     # two steps per pass
+    ((40, 70, 130), 7, 3),     # three steps per pass
+    ((96, 100, 120), 9, 3),
+]
+
+
+@pytest.mark.parametrize("dims,steps,depth", PERSIST_CASES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("boundary", ["clamp", "zero"])
+def test_persistent_matches_oracle(gpu, dims, steps, depth, dtype, boundary):
+    tune = [0, 0, depth] if depth else None
+    run_case("persistent", dims, steps, dtype, boundary, tune=tune)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_persistent_config_coefs(gpu, dtype):
+    run_case("persistent", (48, 80, 160), 10, dtype, "zero", coefs=CONFIG_3D)
+
+
+def test_persistent_result_buffer_parity(gpu):
+    torch = gpu
+    dims = (20, 40, 100)
+    n = int(np.prod(dims))
+    si = P.shape_info("3d_star_r1")
+    for steps in (1, 2, 3, 4, 6):
+        a = torch.empty(n, dtype=torch.float64, device="cuda")
+        b = torch.empty(n, dtype=torch.float64, device="cuda")
+        P.fill_uniform(a, n, 5, 0.0, 1.0)
+        rep = P.j3d_device(a, b, *dims, steps, method="persistent", boundary="clamp")
+        res = (b if rep.result_in_b else a).cpu().numpy().reshape(dims)
+        want = O.iterate(O.uniform(n, 5, 0.0, 1.0).reshape(dims), O.stencil_from_shape(si, "clamp"), steps)
+        assert np.array_equal(res, want), steps
