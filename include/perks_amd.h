@@ -95,7 +95,7 @@ typedef struct perks_stencil_opts {
     int32_t warmup;         /* --warmup: run untimed launches first */
     int32_t warmup_steps;   /* --warmiter */
     /* experimental knobs, 0 = automatic:
-     *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 1 = nontemporal stores
+     *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 2 = cached stores (default nontemporal)
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
      *   tune[4] persistent: tile variant (0..3)      tune[5..7] unused */
     int32_t tune[8];
@@ -178,11 +178,43 @@ int perks_j3d_device_f64(double* d_a, double* d_b, int height, int width_y, int 
 int perks_j3d_device_f32(float* d_a, float* d_b, int height, int width_y, int width_x, int steps,
                          const perks_stencil_opts* opts, void* stream, perks_run_report* report);
 
+/* ---- 2D Jacobi ------------------------------------------------------------ */
+/* Host buffers h_in[width_y][width_x] (x fastest), as the reference's
+ * jacobi_iterative<REAL>.  report may be NULL. */
+int perks_jacobi2d_iterative_f64(const double* h_in, int width_y, int width_x, double* h_out, int steps,
+                                 const perks_stencil_opts* opts, perks_run_report* report);
+int perks_jacobi2d_iterative_f32(const float* h_in, int width_y, int width_x, float* h_out, int steps,
+                                 const perks_stencil_opts* opts, perks_run_report* report);
+/* Device-resident field of any storage type (PERKS_F64, PERKS_F32, and
+ * PERKS_F16 / PERKS_BF16 storage with FP32 arithmetic).  d_a holds the
+ * input; d_a and d_b are both overwritten; the result is in d_b when
+ * report->result_in_b is 1, else in d_a. */
+int perks_jacobi2d_device(int dtype, void* d_a, void* d_b, int width_y, int width_x, int steps,
+                          const perks_stencil_opts* opts, void* stream, perks_run_report* report);
+
+/* Batched 2D Jacobi: many independent grids of any sizes and step counts in
+ * one persistent launch (PERKS_METHOD_PERSISTENT) or as per-step launches
+ * (PERKS_METHOD_PLAIN).  All grids share dtype, shape and coefficients.  On
+ * return each grid's result_in_b tells where its result is. */
+typedef struct perks_grid2d {
+    void* a;
+    void* b;
+    int32_t width_y, width_x;
+    int32_t steps;
+    int32_t result_in_b; /* out */
+} perks_grid2d;
+int perks_jacobi2d_batch(int dtype, perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts, void* stream,
+                         perks_run_report* report);
+
 /* ---- inputs ---------------------------------------------------------------- */
 /* d[i] = lo + (hi-lo) * u53(splitmix64(seed + (i+1)*0x9E3779B97F4A7C15)),
  * the same sequence the host generator produces (float: rounded once). */
 int perks_fill_uniform_f64(double* d, size_t n, uint64_t seed, double lo, double hi, void* stream);
 int perks_fill_uniform_f32(float* d, size_t n, uint64_t seed, double lo, double hi, void* stream);
+/* The f32 sequence above, each value then rounded to binary16 / bfloat16
+ * (nearest even). */
+int perks_fill_uniform_f16(uint16_t* d, size_t n, uint64_t seed, double lo, double hi, void* stream);
+int perks_fill_uniform_bf16(uint16_t* d, size_t n, uint64_t seed, double lo, double hi, void* stream);
 
 #ifdef __cplusplus
 }

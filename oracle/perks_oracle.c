@@ -170,6 +170,93 @@ DEFINE_STEP(f64, double)
 DEFINE_STEP(f32, float)
 
 /* ------------------------------------------------------------------------ */
+/* Reduced-precision storage                                                 */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* IEEE binary32 -> binary16, round to nearest even, with subnormals,
+ * overflow to infinity and NaN kept quiet. */
+uint16_t po_f32_to_f16(float f)
+{
+    const uint32_t u = f2u(f);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t absu = u & 0x7FFFFFFFu;
+    if (absu >= 0x7F800000u) /* inf or nan */
+        return (uint16_t)(sign | 0x7C00u | (absu > 0x7F800000u ? 0x0200u | ((absu >> 13) & 0x3FFu) : 0u));
+    if (absu >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u); /* rounds to >= 65520: inf */
+    if (absu < 0x38800000u) {
+        /* result is subnormal or zero: value = absu_float / 2^-24 rounded */
+        if (absu < 0x33000000u) return (uint16_t)sign; /* < 2^-25: rounds to 0 */
+        const uint32_t e = absu >> 23;                 /* biased exponent */
+        const uint32_t m = (absu & 0x7FFFFFu) | 0x800000u;
+        const uint32_t shift = 126u - e;               /* 14 + (113 - e) */
+        uint32_t r = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (r & 1u))) ++r;
+        return (uint16_t)(sign | r);
+    }
+    /* normal: rebias exponent, round mantissa from 23 to 10 bits */
+    uint32_t r = ((absu - 0x38000000u) >> 13);
+    const uint32_t rem = absu & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+    return (uint16_t)(sign | r);
+}
+
+float po_f16_to_f32(uint16_t h)
+{
+    const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    const uint32_t e = ((uint32_t)h >> 10) & 0x1Fu;
+    const uint32_t m = (uint32_t)h & 0x3FFu;
+    if (e == 0) {
+        if (m == 0) return u2f(sign);
+        return (sign ? -1.0f : 1.0f) * ldexpf((float)m, -24);
+    }
+    if (e == 31) return u2f(sign | 0x7F800000u | (m << 13));
+    return u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+uint16_t po_f32_to_bf16(float f)
+{
+    const uint32_t u = f2u(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u); /* quiet nan */
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+float po_bf16_to_f32(uint16_t b) { return u2f((uint32_t)b << 16); }
+
+static void iterate_u16(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st,
+                        int steps, int bf16)
+{
+    const size_t n = (size_t)nz * ny * nx;
+    float* a = (float*)malloc(n * sizeof(float));
+    float* b = (float*)malloc(n * sizeof(float));
+    for (size_t i = 0; i < n; ++i) a[i] = bf16 ? po_bf16_to_f32(in[i]) : po_f16_to_f32(in[i]);
+    for (int s = 0; s < steps; ++s) {
+        po_stencil_step_f32(a, b, nz, ny, nx, st);
+        for (size_t i = 0; i < n; ++i) {
+            const uint16_t q = bf16 ? po_f32_to_bf16(b[i]) : po_f32_to_f16(b[i]);
+            a[i] = bf16 ? po_bf16_to_f32(q) : po_f16_to_f32(q);
+        }
+    }
+    for (size_t i = 0; i < n; ++i) out[i] = bf16 ? po_f32_to_bf16(a[i]) : po_f32_to_f16(a[i]);
+    free(a);
+    free(b);
+}
+
+void po_stencil_iterate_f16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps)
+{
+    iterate_u16(in, out, nz, ny, nx, st, steps, 0);
+}
+
+void po_stencil_iterate_bf16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps)
+{
+    iterate_u16(in, out, nz, ny, nx, st, steps, 1);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Sparse matrices                                                           */
 /* ------------------------------------------------------------------------ */
 

@@ -75,6 +75,16 @@ void po_stencil_iterate_f32(const float* in, float* out, int nz, int ny, int nx,
 void po_stencil_iterate_omp_f64(const double* in, double* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
 void po_stencil_iterate_omp_f32(const float* in, float* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
 
+/* Reduced-precision storage: fields held as IEEE binary16 ("f16s") or bfloat16
+ * ("bf16s") bit patterns, each step computed in float exactly as the f32 path
+ * and rounded to the storage format (round to nearest, ties to even). */
+uint16_t po_f32_to_f16(float f);
+float po_f16_to_f32(uint16_t h);
+uint16_t po_f32_to_bf16(float f);
+float po_bf16_to_f32(uint16_t b);
+void po_stencil_iterate_f16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps);
+void po_stencil_iterate_bf16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps);
+
 /* RMS and max |a-b| over a box [z0,z1)x[y0,y1)x[x0,x1) (common.hpp:133-163). */
 double po_rms_error_f64(const double* a, const double* b, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1, double* max_err);
 double po_rms_error_f32(const float* a, const float* b, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1, double* max_err);

@@ -6,10 +6,11 @@ entry points; it loads the library and fails loudly if it is missing.
 """
 from . import _lib
 from ._lib import PerksError, check, load
-from .stencil import (fill_uniform, j3d_device, j3d_iterative, j3d_run, make_opts, plan, query_device, shape_id,
-                      shape_info)
+from .stencil import (fill_uniform, j3d_device, j3d_iterative, j3d_run, jacobi2d_batch, jacobi2d_device,
+                      jacobi2d_run, jacobi_iterative, make_opts, plan, query_device, shape_id, shape_info)
 
 __all__ = [
-    "PerksError", "check", "load", "fill_uniform", "j3d_device", "j3d_iterative", "j3d_run", "make_opts", "plan",
-    "query_device", "shape_id", "shape_info", "_lib",
+    "PerksError", "check", "load", "fill_uniform", "j3d_device", "j3d_iterative", "j3d_run", "jacobi2d_batch",
+    "jacobi2d_device", "jacobi2d_run", "jacobi_iterative", "make_opts", "plan", "query_device", "shape_id",
+    "shape_info", "_lib",
 ]

@@ -79,6 +79,11 @@ class PlanInfo(C.Structure):
     ]
 
 
+class Grid2d(C.Structure):
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("width_y", C.c_int32), ("width_x", C.c_int32),
+                ("steps", C.c_int32), ("result_in_b", C.c_int32)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -101,6 +106,12 @@ SIGNATURES = {
     "perks_j3d_device_f32": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
     "perks_fill_uniform_f64": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
     "perks_fill_uniform_f32": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
+    "perks_fill_uniform_f16": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
+    "perks_fill_uniform_bf16": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
+    "perks_jacobi2d_iterative_f64": (_I, [_P, _I, _I, _P, _I, C.POINTER(StencilOpts), C.POINTER(RunReport)]),
+    "perks_jacobi2d_iterative_f32": (_I, [_P, _I, _I, _P, _I, C.POINTER(StencilOpts), C.POINTER(RunReport)]),
+    "perks_jacobi2d_device": (_I, [_I, _P, _P, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
+    "perks_jacobi2d_batch": (_I, [_I, C.POINTER(Grid2d), _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
 }
 
 _lib = None

@@ -16,6 +16,7 @@
 #include "j3d_kernels.h"
 #include "j3d_persistent.h"
 #include "planner.h"
+#include "j2d.h"
 
 namespace perks {
 
@@ -49,7 +50,7 @@ int run_plain(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& c
     rc = plan_plain(nz, ny, nx, caps.cu_count, &pl);
     if (rc != PERKS_OK) return rc;
     const int zchunk = o->tune[0] > 0 ? o->tune[0] : pl.reserved[0];
-    const int variant = o->tune[1] ? 1 : 0;
+    const int variant = o->tune[1] == 2 ? 0 : 1;  // nontemporal stores unless asked otherwise
     const int bc = o->boundary;
     hipEvent_t ev0, ev1;
     PERKS_HIP_TRY(hipEventCreate(&ev0));
@@ -233,7 +234,12 @@ int perks_shape_info(int shape, int32_t* dims, int32_t* radius, int32_t* ngroups
     return PERKS_OK;
 }
 
-int perks_release(void) { return perks::release_persistent_scratch(); }
+int perks_release(void)
+{
+    const int a = perks::release_persistent_scratch();
+    const int b = perks::release_j2d_scratch();
+    return a != PERKS_OK ? a : b;
+}
 
 int perks_j3d_iterative_f64(const double* h_in, int height, int width_y, int width_x, double* h_out, int steps,
                             const perks_stencil_opts* opts, perks_run_report* report)

@@ -1,0 +1,49 @@
+// j2d.h -- 2D Jacobi (single grid and batches of independent grids).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "j3d_common.h"
+
+namespace perks {
+
+// One grid of a run: buffers a (input) and b, both overwritten; the result is
+// in b when result_in_b is set on return, else in a.
+struct J2dSpec {
+    void* a;
+    void* b;
+    int ny, nx;
+    int steps;
+    int result_in_b;  // out
+};
+
+// Device-side descriptor (sorted by pass count, item_end cumulative).
+struct J2dGrid {
+    void* a;
+    void* b;
+    int ny, nx;
+    int steps;
+    int tiles;
+    int item_end;
+};
+
+struct J2dArgs {
+    const J2dGrid* grids;
+    const int* active;      // grids still running in each pass (a prefix of grids)
+    int npass;              // passes this launch runs
+    int step_offset;        // index of this launch's first pass (per-step launches)
+    int levels;             // time steps per full pass
+    unsigned int* barrier;  // grid barrier counter, zeroed before the launch
+};
+
+int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);
+int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o);
+int release_j2d_scratch();
+
+// Kernel entry points per storage type (j2d_<type>.hip); nullptr if not built.
+void* j2d_kernel_f64(int shape, int bc, bool has_div);
+void* j2d_kernel_f32(int shape, int bc, bool has_div);
+void* j2d_kernel_f16(int shape, int bc, bool has_div);
+void* j2d_kernel_bf16(int shape, int bc, bool has_div);
+
+}  // namespace perks

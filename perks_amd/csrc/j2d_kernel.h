@@ -1,0 +1,272 @@
+// j2d_kernel.h -- the 2D Jacobi kernel template (instantiated per storage type
+// in j2d_f64.hip, j2d_f32.hip, j2d_f16.hip, j2d_bf16.hip so they build in parallel).
+//
+// A workgroup stages a TX x TY tile of the field, including a ghost rim of
+// R*L cells (R = shape radius, L = time steps in the pass), in LDS, runs the L
+// steps in LDS (ping-pong between two LDS tiles, one workgroup barrier per
+// step), and writes back the tile's interior.  Arithmetic follows the shape
+// table in order (no FMA contraction); FP16/BF16 storage is widened to FP32
+// for the update and rounded to nearest-even on store.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "j2d.h"
+
+namespace perks {
+namespace j2dk {
+
+// ---- storage formats --------------------------------------------------------
+struct f16s {
+    uint16_t v;
+};
+struct bf16s {
+    uint16_t v;
+};
+
+template <typename S>
+struct Fmt;
+template <>
+struct Fmt<double> {
+    using C = double;
+    __device__ static C ld(const double* p, size_t i) { return p[i]; }
+    __device__ static void st(double* p, size_t i, C v) { p[i] = v; }
+    __device__ static C q(C v) { return v; }
+};
+template <>
+struct Fmt<float> {
+    using C = float;
+    __device__ static C ld(const float* p, size_t i) { return p[i]; }
+    __device__ static void st(float* p, size_t i, C v) { p[i] = v; }
+    __device__ static C q(C v) { return v; }
+};
+template <>
+struct Fmt<f16s> {
+    using C = float;
+    __device__ static C ld(const f16s* p, size_t i) { return __half2float(__ushort_as_half(p[i].v)); }
+    __device__ static void st(f16s* p, size_t i, C v) { p[i].v = __half_as_ushort(__float2half_rn(v)); }
+    // The value after a round trip through storage (every step is stored).
+    __device__ static C q(C v) { return __half2float(__float2half_rn(v)); }
+};
+template <>
+struct Fmt<bf16s> {
+    using C = float;
+    __device__ static C ld(const bf16s* p, size_t i) { return __uint_as_float((uint32_t)p[i].v << 16); }
+    // Round to nearest even; NaN stays a quiet NaN.
+    __device__ static uint16_t bits(C v)
+    {
+        const uint32_t u = __float_as_uint(v);
+        return ((u & 0x7FFFFFFFu) > 0x7F800000u) ? (uint16_t)((u >> 16) | 0x40u)
+                                                 : (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    }
+    __device__ static void st(bf16s* p, size_t i, C v) { p[i].v = bits(v); }
+    __device__ static C q(C v) { return __uint_as_float((uint32_t)bits(v) << 16); }
+};
+
+// ---- one cell update from the shape table ------------------------------------
+template <int SHAPE>
+constexpr const ShapeDesc& shape_desc()
+{
+    return SHAPE == PERKS_SHAPE_2D_STAR_R1   ? kShape2dStarR1
+           : SHAPE == PERKS_SHAPE_2D_STAR_R2 ? kShape2dStarR2
+           : SHAPE == PERKS_SHAPE_2D_STAR_R3 ? kShape2dStarR3
+           : SHAPE == PERKS_SHAPE_2D_STAR_R4 ? kShape2dStarR4
+           : SHAPE == PERKS_SHAPE_2D_STAR_R5 ? kShape2dStarR5
+           : SHAPE == PERKS_SHAPE_2D_STAR_R6 ? kShape2dStarR6
+           : SHAPE == PERKS_SHAPE_2D_BOX_R1  ? kShape2dBoxR1
+                                             : kShape2dBoxR2;
+}
+
+template <int SHAPE>
+constexpr int first_f64_group()
+{
+    constexpr const ShapeDesc& s = shape_desc<SHAPE>();
+    for (int g = 0; g < s.ngroups; ++g)
+        if (s.g[g].prod_f64) return g;
+    return s.ngroups;
+}
+
+// Evaluates the shape at one cell.  `get(dy, dx)` returns the neighbour.
+// Groups before the first double-literal group accumulate in C; from that
+// group on (only possible when C is float) the sum is carried in double.
+template <int SHAPE, typename C, bool HAS_DIV, typename Get>
+__device__ __forceinline__ C eval_shape(const C (&k)[kMaxGroups], double kd_div, const double (&kd)[kMaxGroups],
+                                        Get&& get)
+{
+    constexpr const ShapeDesc& S = shape_desc<SHAPE>();
+    constexpr int P0 = sizeof(C) == 8 ? S.ngroups : first_f64_group<SHAPE>();
+    C acc = 0;
+    double accd = 0.0;
+#pragma unroll
+    for (int g = 0; g < S.ngroups; ++g) {
+        C s = get(S.g[g].off[0][1], S.g[g].off[0][2]);
+#pragma unroll
+        for (int p = 1; p < S.g[g].npts; ++p) s = s + get(S.g[g].off[p][1], S.g[g].off[p][2]);
+        if (g < P0) {
+            const C prod = k[g] * s;
+            acc = g == 0 ? prod : acc + prod;
+        } else if (S.g[g].prod_f64) {
+            const double prod = kd[g] * (double)s;
+            accd = g == 0 ? prod : (g == P0 ? (double)acc + prod : accd + prod);
+        } else {
+            const C prod = k[g] * s;
+            accd = accd + (double)prod;
+        }
+    }
+    if (P0 < S.ngroups) return (C)(HAS_DIV ? accd / kd_div : accd);
+    return HAS_DIV ? acc / (C)kd_div : acc;
+}
+
+// ---- the kernel ---------------------------------------------------------------
+__device__ __forceinline__ void grid_barrier2(unsigned int* counter, unsigned int nblocks, unsigned int epoch)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int target = epoch * nblocks;
+        while ((int)(__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0)
+            __builtin_amdgcn_s_sleep(2);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+template <int SHAPE, typename S, int BC, int TX, int TY, bool HAS_DIV>
+__global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
+{
+    using C = typename Fmt<S>::C;
+    constexpr const ShapeDesc& SD = shape_desc<SHAPE>();
+    constexpr int R = SD.radius;
+    __shared__ C tile[2][TY][TX];
+
+    C k[kMaxGroups];
+    double kd[kMaxGroups];
+#pragma unroll
+    for (int g = 0; g < kMaxGroups; ++g) {
+        k[g] = (C)cf.c[g];
+        kd[g] = cf.c[g];
+    }
+    const double kdiv = cf.divisor;
+
+    const int tid = threadIdx.x;
+    const int G = gridDim.x;
+    const int me = xcd_tile_id(blockIdx.x, G);
+
+    for (int pass = 0; pass < a.npass; ++pass) {
+        const int gp = pass + a.step_offset;  // global pass index (per-step launches pass one each)
+        const int ngrids = a.active[pass];
+        const int nitems = a.grids[ngrids - 1].item_end;
+        // Tile geometry always uses the full pass depth, so the tiling (and the
+        // host's item count) is the same in every pass; a shorter last pass
+        // simply runs fewer levels inside the same ghost rim.
+        const int RL = R * a.levels;
+        const int VW = TX - 2 * RL, VH = TY - 2 * RL;
+        for (int item = me; item < nitems; item += G) {
+            int lo = 0, hi = ngrids - 1;  // first grid whose item_end > item
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (a.grids[mid].item_end > item) hi = mid;
+                else lo = mid + 1;
+            }
+            const J2dGrid gd = a.grids[lo];
+            const int L = min(a.levels, gd.steps - gp * a.levels);
+            const int local = item - (gd.item_end - gd.tiles);
+            const int ntx = (gd.nx + VW - 1) / VW;
+            const int tx = local % ntx, ty = local / ntx;
+            const int vx0 = tx * VW, vy0 = ty * VH;
+            const int ox = vx0 - RL, oy = vy0 - RL;  // domain coordinates of tile cell (0,0)
+            const S* src = (const S*)((gp & 1) ? gd.b : gd.a);
+            S* dst = (S*)((gp & 1) ? gd.a : gd.b);
+            const int nx = gd.nx, ny = gd.ny;
+
+            __syncthreads();  // the previous item may still be reading the tile
+            for (int i = tid; i < TX * TY; i += 256) {
+                const int ly = i / TX, lx = i % TX;
+                const int y = min(max(oy + ly, 0), ny - 1);
+                const int x = min(max(ox + lx, 0), nx - 1);
+                tile[0][ly][lx] = Fmt<S>::ld(src, (size_t)y * nx + x);
+            }
+            __syncthreads();
+            // Domain edges in tile coordinates; tiles that reach past an edge
+            // clamp (or zero) the neighbour reads, interior tiles read directly.
+            const int dxlo = -ox, dxhi = nx - 1 - ox, dylo = -oy, dyhi = ny - 1 - oy;
+            const bool edge = dxlo > 0 || dylo > 0 || dxhi < TX - 1 || dyhi < TY - 1;
+            for (int l = 0; l < L; ++l) {
+                const int cur = l & 1;
+                for (int i = tid; i < (TX - 2 * R) * (TY - 2 * R); i += 256) {
+                    const int ly = R + i / (TX - 2 * R), lx = R + i % (TX - 2 * R);
+                    C v;
+                    if (!edge) {
+                        v = eval_shape<SHAPE, C, HAS_DIV>(k, kdiv, kd, [&](int dy, int dx) {
+                            return tile[cur][ly + dy][lx + dx];
+                        });
+                    } else {
+                        v = eval_shape<SHAPE, C, HAS_DIV>(k, kdiv, kd, [&](int dy, int dx) {
+                            int yy = ly + dy, xx = lx + dx;
+                            if (BC == PERKS_BC_ZERO) {
+                                const bool out = yy < dylo || yy > dyhi || xx < dxlo || xx > dxhi;
+                                return out ? (C)0 : tile[cur][yy][xx];
+                            }
+                            yy = min(max(yy, dylo), dyhi);
+                            xx = min(max(xx, dxlo), dxhi);
+                            return tile[cur][yy][xx];
+                        });
+                    }
+                    // Each step is a stored field: round to the storage format.
+                    tile[cur ^ 1][ly][lx] = Fmt<S>::q(v);
+                }
+                __syncthreads();
+            }
+            const int fin = L & 1;
+            for (int i = tid; i < VW * VH; i += 256) {
+                const int ly = RL + i / VW, lx = RL + i % VW;
+                const int y = oy + ly, x = ox + lx;
+                if (y < ny && x < nx) Fmt<S>::st(dst, (size_t)y * nx + x, tile[fin][ly][lx]);
+            }
+        }
+        if (pass + 1 < a.npass) grid_barrier2(a.barrier, G, (unsigned int)(pass + 1));
+    }
+}
+
+template <typename S>
+struct TileOf {
+    static constexpr int TX = 128, TY = 64;
+};
+template <>
+struct TileOf<double> {
+    static constexpr int TX = 64, TY = 64;
+};
+
+template <int SHAPE, typename S, int BC, bool HD>
+void* kptr3()
+{
+    return reinterpret_cast<void*>(&j2d_kernel<SHAPE, S, BC, TileOf<S>::TX, TileOf<S>::TY, HD>);
+}
+
+template <int SHAPE, typename S>
+void* kptr2(int bc, bool hd)
+{
+    if (bc == PERKS_BC_ZERO) return hd ? kptr3<SHAPE, S, PERKS_BC_ZERO, true>() : kptr3<SHAPE, S, PERKS_BC_ZERO, false>();
+    return hd ? kptr3<SHAPE, S, PERKS_BC_CLAMP, true>() : kptr3<SHAPE, S, PERKS_BC_CLAMP, false>();
+}
+
+template <typename S>
+void* kptr(int shape, int bc, bool hd)
+{
+    switch (shape) {
+    case PERKS_SHAPE_2D_STAR_R1: return kptr2<PERKS_SHAPE_2D_STAR_R1, S>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R2: return kptr2<PERKS_SHAPE_2D_STAR_R2, S>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R3: return kptr2<PERKS_SHAPE_2D_STAR_R3, S>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R4: return kptr2<PERKS_SHAPE_2D_STAR_R4, S>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R5: return kptr2<PERKS_SHAPE_2D_STAR_R5, S>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R6: return kptr2<PERKS_SHAPE_2D_STAR_R6, S>(bc, hd);
+    case PERKS_SHAPE_2D_BOX_R1: return kptr2<PERKS_SHAPE_2D_BOX_R1, S>(bc, hd);
+    case PERKS_SHAPE_2D_BOX_R2: return kptr2<PERKS_SHAPE_2D_BOX_R2, S>(bc, hd);
+    default: return nullptr;
+    }
+}
+
+}  // namespace j2dk
+}  // namespace perks

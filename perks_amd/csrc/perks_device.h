@@ -118,6 +118,16 @@ __device__ __forceinline__ float keep_or_zero(float v, bool keep)
     return __int_as_float(__float_as_int(v) & m);
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for every
+// outstanding global load (s_waitcnt vmcnt(0)), which would drain the
+// prefetch queue at each exchange; this waits for LDS operations only.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Workgroup id remapped so that consecutive tiles share an XCD.  The
 // dispatcher hands workgroup L to XCD L % 8; giving XCD k the contiguous tile
 // range [k*G/8, (k+1)*G/8) keeps y/z neighbours (which re-read each other's

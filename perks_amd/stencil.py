@@ -167,9 +167,17 @@ def _ptr_stream(stream) -> int:
     raise TypeError("expected a hipStream_t (int) or a torch stream")
 
 
+DTYPES = {"f64": L.F64, "f32": L.F32, "f16": L.F16, "bf16": L.BF16}
+
+
+def _dt(dtype) -> int:
+    return DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
+
+
 def fill_uniform(d, n, seed, lo, hi, dtype="f64", stream=None) -> None:
     lib = L.load()
-    fn = lib.perks_fill_uniform_f64 if dtype in ("f64", L.F64) else lib.perks_fill_uniform_f32
+    fn = {L.F64: lib.perks_fill_uniform_f64, L.F32: lib.perks_fill_uniform_f32, L.F16: lib.perks_fill_uniform_f16,
+          L.BF16: lib.perks_fill_uniform_bf16}[_dt(dtype)]
     s = None if stream is None else _ptr_stream(stream)
     L.check(fn(_ptr(d), int(n), int(seed), float(lo), float(hi), s), "fill_uniform")
 
@@ -189,3 +197,60 @@ def plan(height, width_y, width_x, dtype="f64", caps: L.DeviceCaps | None = None
     rc = L.load().perks_j3d_plan(dt, height, width_y, width_x, C.byref(o), C.byref(caps) if caps else None,
                                  int(occupancy), C.byref(info))
     return rc, info
+
+
+# ---- 2D ---------------------------------------------------------------------------
+def jacobi_iterative(h_input, width_y, width_x, h_output, bdimx=256, blkpsm=0, iteration=3, async_=False,
+                     useSM=False, usewarmup=False, warmupiteration=-1, isDoubleTile=False, *, shape="2d_star_r1",
+                     boundary="clamp", method="persistent", coefs=None, divisor=None,
+                     report: L.RunReport | None = None, **tuning) -> int:
+    """The reference's jacobi_iterative<REAL> (stencil/2dstencil/common/jacobi_cuda.cuh:6-10,
+    jacobi-2d.cu:369-374): h_input[width_y][width_x] -> h_output after
+    `iteration` steps; returns 0 or a negative planner code."""
+    opts = make_opts(shape, boundary, method, coefs, divisor, bdimx=bdimx, blkpsm=blkpsm,
+                     use_lds_cache=1 if useSM else -1, double_tile=int(bool(isDoubleTile)),
+                     warmup=int(bool(usewarmup)), warmup_steps=max(0, int(warmupiteration)), **tuning)
+    for a in (h_input, h_output):
+        if not (isinstance(a, np.ndarray) and a.flags.c_contiguous and a.size == width_y * width_x):
+            raise ValueError("h_input/h_output must be C-contiguous arrays of width_y*width_x elements")
+    if h_input.dtype != h_output.dtype:
+        raise TypeError("h_input and h_output must share a dtype")
+    rep = report if report is not None else L.RunReport()
+    fn = _dtype_fn(h_input, "jacobi2d_iterative")
+    return fn(h_input.ctypes.data, width_y, width_x, h_output.ctypes.data, int(iteration), C.byref(opts),
+              C.byref(rep))
+
+
+def jacobi2d_run(field: np.ndarray, steps: int, **kw) -> tuple[np.ndarray, L.RunReport]:
+    field = np.ascontiguousarray(field)
+    ny, nx = field.shape
+    out = np.empty_like(field)
+    rep = L.RunReport()
+    L.check(jacobi_iterative(field, ny, nx, out, iteration=steps, report=rep, **kw), "jacobi_iterative")
+    return out, rep
+
+
+def jacobi2d_device(d_a, d_b, width_y, width_x, steps, dtype="f64", stream=None, opts=None, **kw) -> L.RunReport:
+    """Device buffers (torch tensors or pointers) of f64/f32/f16/bf16 storage;
+    the result is in d_b when report.result_in_b, else in d_a."""
+    o = opts if opts is not None else make_opts(**kw)
+    rep = L.RunReport()
+    s = None if stream is None else _ptr_stream(stream)
+    L.check(L.load().perks_jacobi2d_device(_dt(dtype), _ptr(d_a), _ptr(d_b), width_y, width_x, int(steps), C.byref(o),
+                                           s, C.byref(rep)), "jacobi2d_device")
+    return rep
+
+
+def jacobi2d_batch(grids, dtype="f32", stream=None, opts=None, **kw):
+    """grids: list of (d_a, d_b, width_y, width_x, steps).  Returns (report,
+    list of result_in_b flags)."""
+    o = opts if opts is not None else make_opts(**kw)
+    arr = (L.Grid2d * len(grids))()
+    for i, (a, b, wy, wx, st) in enumerate(grids):
+        arr[i].a, arr[i].b = _ptr(a), _ptr(b)
+        arr[i].width_y, arr[i].width_x, arr[i].steps = int(wy), int(wx), int(st)
+    rep = L.RunReport()
+    s = None if stream is None else _ptr_stream(stream)
+    L.check(L.load().perks_jacobi2d_batch(_dt(dtype), arr, len(grids), C.byref(o), s, C.byref(rep)),
+            "jacobi2d_batch")
+    return rep, [arr[i].result_in_b for i in range(len(grids))]

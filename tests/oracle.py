@@ -52,6 +52,12 @@ def lib() -> C.CDLL:
             "po_stencil_iterate_f32": (None, [P, P, I, I, I, P, I]),
             "po_stencil_iterate_omp_f64": (None, [P, P, I, I, I, P, I, I]),
             "po_stencil_iterate_omp_f32": (None, [P, P, I, I, I, P, I, I]),
+            "po_f32_to_f16": (C.c_uint16, [C.c_float]),
+            "po_f16_to_f32": (C.c_float, [C.c_uint16]),
+            "po_f32_to_bf16": (C.c_uint16, [C.c_float]),
+            "po_bf16_to_f32": (C.c_float, [C.c_uint16]),
+            "po_stencil_iterate_f16s": (None, [P, P, I, I, I, P, I]),
+            "po_stencil_iterate_bf16s": (None, [P, P, I, I, I, P, I]),
             "po_rms_error_f64": (D, [P, P, I, I, I, I, I, I, I, I, P]),
             "po_rms_error_f32": (D, [P, P, I, I, I, I, I, I, I, I, P]),
             "po_poisson7_nnz": (C.c_int64, [I, I, I]),
@@ -128,6 +134,32 @@ def iterate(field: np.ndarray, st: PoStencil, steps: int, threads: int = 1) -> n
         fn = lib().po_stencil_iterate_f64 if f64 else lib().po_stencil_iterate_f32
         fn(p(field), p(out), nz, ny, nx, C.byref(st), steps)
     return out
+
+
+def iterate16(field_bits: np.ndarray, st: PoStencil, steps: int, fmt: str) -> np.ndarray:
+    """Reduced-precision storage: uint16 bit patterns of binary16 ('f16') or
+    bfloat16 ('bf16'); each step in float, rounded to the format on store."""
+    field_bits = np.ascontiguousarray(field_bits, dtype=np.uint16)
+    if field_bits.ndim == 2:
+        nz, (ny, nx) = 1, field_bits.shape
+    else:
+        nz, ny, nx = field_bits.shape
+    out = np.empty_like(field_bits)
+    fn = lib().po_stencil_iterate_f16s if fmt == "f16" else lib().po_stencil_iterate_bf16s
+    fn(p(field_bits), p(out), nz, ny, nx, C.byref(st), steps)
+    return out
+
+
+def to16(x: np.ndarray, fmt: str) -> np.ndarray:
+    f = lib().po_f32_to_f16 if fmt == "f16" else lib().po_f32_to_bf16
+    flat = np.asarray(x, dtype=np.float32).ravel()
+    return np.array([f(float(v)) for v in flat], dtype=np.uint16).reshape(np.shape(x))
+
+
+def from16(bits: np.ndarray, fmt: str) -> np.ndarray:
+    if fmt == "bf16":
+        return (np.asarray(bits, dtype=np.uint32) << 16).view(np.float32)
+    return np.asarray(bits, dtype=np.uint16).view(np.float16).astype(np.float32)
 
 
 def rms_error(a: np.ndarray, b: np.ndarray, box=None) -> tuple[float, float]:

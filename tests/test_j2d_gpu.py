@@ -1,0 +1,121 @@
+"""2D Jacobi on the GPU through the C ABI, bit for bit against the oracle.
+
+Every 2D shape of the reference (stars r1..r6, boxes r1 and r2) with its
+reference weights -- including the FP32 box shapes whose centre weight the
+reference writes as a double literal -- under both boundary rules, by the
+per-step and the persistent method; FP16/BF16 storage against the oracle's
+reduced-precision path; and batches of independent grids.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import perks_amd as P
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = ["2d5pt", "2ds9pt", "2d13pt", "2d17pt", "2d21pt", "2ds25pt", "2d9pt", "2d25pt"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check_equal(got, want, what):
+    if not np.array_equal(got.view(np.uint8), want.view(np.uint8)):
+        bad = np.argwhere(got != want)
+        rms, mx = O.rms_error(got.astype(np.float64), want.astype(np.float64))
+        pytest.fail(f"{what}: {len(bad)} cells differ, first {bad[:3].tolist()}, rms {rms:.3e} max {mx:.3e}")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("method", ["plain", "persistent"])
+@pytest.mark.parametrize("boundary", ["clamp", "zero"])
+def test_2d_shapes_match_oracle(gpu, shape, dtype, method, boundary):
+    dims, steps = (70, 150), 11
+    say(f"[j2d] {shape} {np.dtype(dtype).name} {method} {boundary} dims={dims} steps={steps}")
+    si = P.shape_info(shape)
+    field = O.libc_rand(dims[0] * dims[1], 3, dtype).reshape(dims)
+    got, rep = P.jacobi2d_run(field, steps, shape=shape, method=method, boundary=boundary)
+    want = O.iterate(field, O.stencil_from_shape(si, boundary), steps)
+    check_equal(got, want, f"{shape} {method} {boundary}")
+
+
+@pytest.mark.parametrize("dims,steps,depth", [((1, 1), 3, 0), ((5, 300), 9, 0), ((257, 33), 17, 0),
+                                               ((200, 200), 20, 3), ((131, 262), 25, 8), ((64, 64), 1, 0)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_2d_sizes_and_pass_depths(gpu, dims, steps, depth, dtype):
+    say(f"[j2d] sizes dims={dims} steps={steps} depth={depth} {np.dtype(dtype).name}")
+    coefs = [0.2] * 5
+    si = P.shape_info("2d5pt")
+    field = O.uniform(dims[0] * dims[1], 4, 0.0, 1.0, dtype).reshape(dims)
+    tune = [0, 0, depth] if depth else None
+    got, _ = P.jacobi2d_run(field, steps, shape="2d5pt", method="persistent", boundary="zero", coefs=coefs,
+                            tune=tune)
+    want = O.iterate(field, O.stencil_from_shape(si, "zero", coefs=coefs), steps)
+    check_equal(got, want, f"dims {dims}")
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+@pytest.mark.parametrize("shape", ["2d9pt", "2d25pt"])
+@pytest.mark.parametrize("method", ["plain", "persistent"])
+def test_2d_reduced_precision_storage(gpu, fmt, shape, method):
+    torch = gpu
+    dims, steps = (90, 200), 13
+    say(f"[j2d] storage {fmt} {shape} {method}")
+    si = P.shape_info(shape)
+    rng = np.random.default_rng(5)
+    c = rng.random(len(si.coef))
+    coefs = list(c / c.sum())
+    n = dims[0] * dims[1]
+    a = torch.empty(n, dtype=torch.int16, device="cuda")
+    b = torch.empty(n, dtype=torch.int16, device="cuda")
+    P.fill_uniform(a, n, 21, -1.0, 1.0, dtype=fmt)
+    init = a.cpu().numpy().view(np.uint16).reshape(dims).copy()
+    want_init = O.to16(O.uniform(n, 21, -1.0, 1.0, np.float32), fmt).reshape(dims)
+    assert np.array_equal(init, want_init), "device 16-bit fill differs from the host sequence"
+    rep = P.jacobi2d_device(a, b, dims[0], dims[1], steps, dtype=fmt, shape=shape, method=method, boundary="clamp",
+                            coefs=coefs)
+    got = (b if rep.result_in_b else a).cpu().numpy().view(np.uint16).reshape(dims)
+    want = O.iterate16(init, O.stencil_from_shape(si, "clamp", coefs=coefs), steps, fmt)
+    check_equal(got, want, f"{fmt} {shape}")
+
+
+@pytest.mark.parametrize("dtype,tdt", [("f32", "float32"), ("f64", "float64")])
+def test_2d_batch_matches_per_grid_oracle(gpu, dtype, tdt):
+    torch = gpu
+    rng = np.random.default_rng(11)
+    specs = []
+    for i in range(12):
+        wy, wx = int(rng.integers(20, 180)), int(rng.integers(20, 260))
+        steps = int(rng.integers(1, 30))
+        specs.append((wy, wx, steps))
+    say(f"[j2d] batch {dtype} {specs}")
+    si = P.shape_info("2d9pt")
+    coefs = [0.1, 0.1, 0.1, 0.1, 0.2, 0.1, 0.1, 0.1, 0.1]
+    bufs, fields = [], []
+    for i, (wy, wx, st) in enumerate(specs):
+        f = O.uniform(wy * wx, 100 + i, 0.0, 1.0, np.float64 if dtype == "f64" else np.float32).reshape(wy, wx)
+        a = torch.from_numpy(f.copy()).cuda()
+        b = torch.empty_like(a)
+        bufs.append((a, b))
+        fields.append(f)
+    rep, flags = P.jacobi2d_batch([(a, b, wy, wx, st) for (a, b), (wy, wx, st) in zip(bufs, specs)], dtype=dtype,
+                                  shape="2d9pt", method="persistent", boundary="zero", coefs=coefs)
+    for i, ((a, b), (wy, wx, st)) in enumerate(zip(bufs, specs)):
+        got = (b if flags[i] else a).cpu().numpy()
+        want = O.iterate(fields[i], O.stencil_from_shape(si, "zero", coefs=coefs), st)
+        check_equal(got, want, f"grid {i} {wy}x{wx} steps {st}")
+
+
+def test_2d_reference_signature(gpu):
+    field = O.libc_rand(64 * 128, 1, np.float64).reshape(64, 128)
+    out = np.empty_like(field)
+    rc = P.jacobi_iterative(field, 64, 128, out, 256, 0, 5, False, False, False, -1, False)
+    assert rc == 0
+    want = O.iterate(field, O.stencil_from_shape(P.shape_info("2d5pt"), "clamp"), 5)
+    check_equal(out, want, "jacobi_iterative")
