@@ -206,6 +206,73 @@ typedef struct perks_grid2d {
 int perks_jacobi2d_batch(int dtype, perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts, void* stream,
                          perks_run_report* report);
 
+/* ---- conjugate gradient ---------------------------------------------------- */
+/* Device-resident CSR: rowptr[n+1], col[nnz], val[nnz] (columns ascending
+ * within a row, as the reference's CSR built by sparse_matrix.h:666-734). */
+typedef struct perks_csr {
+    int32_t n;
+    int32_t reserved0;
+    int64_t nnz;
+    const int32_t* rowptr;
+    const int32_t* col;
+    const double* val;
+} perks_csr;
+
+/* DIA: ndiag (<= 16) diagonals; offsets is a HOST array, val a device array
+ * of ndiag*n values with val[d*n + i] = A[i][i + offsets[d]] (entries whose
+ * column falls outside [0,n) are ignored). */
+typedef struct perks_dia {
+    int32_t n;
+    int32_t ndiag;
+    const int32_t* offsets;
+    const double* val;
+} perks_dia;
+
+enum perks_cg_precision { PERKS_CG_F64 = 0, PERKS_CG_F32_REFINE = 1 };
+
+typedef struct perks_cg_opts {
+    double tol;          /* stop when r.r < tol^2 (* b.b when relative)      */
+    double inner_tol;    /* F32_REFINE: relative tolerance of each inner solve */
+    int32_t relative;    /* 1: ||r||/||b|| < tol ; 0: ||r|| < tol (cg_kernels.cuh:834) */
+    int32_t max_iter;    /* iterations (F64) / per inner solve (F32_REFINE) */
+    int32_t precision;   /* enum perks_cg_precision */
+    int32_t max_inner;   /* F32_REFINE: cap per inner solve (0 = max_iter) */
+    int32_t max_outer;   /* F32_REFINE: refinement steps (0 = 20)         */
+    int32_t x0_is_zero;  /* 1: skip the initial A x (x must hold zeros)    */
+    int32_t blkpsm;      /* cap on workgroups per CU (0 = occupancy)       */
+    int32_t reserved[5];
+} perks_cg_opts;
+
+typedef struct perks_cg_report {
+    int32_t iterations;       /* CG iterations (summed over inner solves)    */
+    int32_t outer_iterations; /* refinement steps (F32_REFINE)              */
+    double final_rr;          /* r.r of the FP64 residual at exit            */
+    double bb;                /* b.b                                         */
+    double elapsed_ms;        /* device time of the solve (hipEvents)        */
+    int32_t grid;             /* workgroups of the persistent launch         */
+    int32_t hist_cap;         /* capacity of history (caller-provided)       */
+    double* history;          /* host array: r.r after each iteration (F64:   */
+                              /* [0] initial) or each refinement step        */
+    int32_t hist_len;         /* entries produced (may exceed hist_cap)      */
+    int32_t reserved[5];
+} perks_cg_report;
+
+void perks_cg_opts_init(perks_cg_opts* opts);
+/* Solves A x = b; d_b, d_x device vectors of n doubles (x is the initial
+ * guess and receives the solution).  Returns PERKS_OK when converged,
+ * PERKS_ERR_NOT_CONVERGED when max_iter ran out (x and report are valid). */
+int perks_cg_solve_csr(const perks_csr* A, const double* d_b, double* d_x, const perks_cg_opts* opts, void* stream,
+                       perks_cg_report* report);
+int perks_cg_solve_dia(const perks_dia* A, const double* d_b, double* d_x, const perks_cg_opts* opts, void* stream,
+                       perks_cg_report* report);
+
+/* 3D Poisson 7-point operator on nx*ny*nz points (x fastest, Dirichlet):
+ * diagonal 6, off-diagonals -1.  CSR arrays and the 7 DIA diagonals (offsets
+ * {-nx*ny, -nx, -1, 0, 1, nx, nx*ny}) are generated on the device. */
+int64_t perks_poisson7_nnz(int nx, int ny, int nz);
+int perks_poisson7_csr(int nx, int ny, int nz, int32_t* d_rowptr, int32_t* d_col, double* d_val, void* stream);
+int perks_poisson7_dia(int nx, int ny, int nz, int32_t* offsets, double* d_val, void* stream);
+
 /* ---- inputs ---------------------------------------------------------------- */
 /* d[i] = lo + (hi-lo) * u53(splitmix64(seed + (i+1)*0x9E3779B97F4A7C15)),
  * the same sequence the host generator produces (float: rounded once). */

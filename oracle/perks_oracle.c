@@ -453,3 +453,80 @@ int po_cg_f64(int fmt, int n, const int32_t* rowptr, const int32_t* col, const d
     free(r); free(p); free(Ap);
     return k - 1;
 }
+
+/* ---- FP32-inner / FP64-refinement CG (see perks_oracle.h) ---------------- */
+static void spmv_f32(int fmt, int n, const int32_t* rowptr, const int32_t* col, const float* val, int ndiag,
+                     const float* x, float* y)
+{
+    for (int i = 0; i < n; ++i) {
+        float s = 0.0f;
+        if (fmt == 0) {
+            for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * x[col[k]];
+        } else {
+            for (int d = 0; d < ndiag; ++d) {
+                int64_t j = (int64_t)i + col[d];
+                if (j < 0 || j >= n) continue;
+                s = s + val[(size_t)d * n + i] * x[j];
+            }
+        }
+        y[i] = s;
+    }
+}
+
+int po_cg_f32_refine(int fmt, int n, const int32_t* rowptr, const int32_t* col, const double* val, int ndiag,
+                     const double* b, double* x, double tol, int relative, double inner_tol, int max_inner,
+                     int max_outer, double* outer_hist, int hist_cap, int* outer_out)
+{
+    const size_t nval = fmt == 0 ? (size_t)rowptr[n] : (size_t)ndiag * n;
+    float* valf = (float*)malloc(nval * sizeof(float));
+    for (size_t i = 0; i < nval; ++i) valf[i] = (float)val[i];
+    double* r = (double*)malloc((size_t)n * sizeof(double));
+    double* Ax = (double*)malloc((size_t)n * sizeof(double));
+    float* rf = (float*)malloc((size_t)n * sizeof(float));
+    float* bf = (float*)malloc((size_t)n * sizeof(float));
+    float* d = (float*)malloc((size_t)n * sizeof(float));
+    float* p = (float*)malloc((size_t)n * sizeof(float));
+    float* Ap = (float*)malloc((size_t)n * sizeof(float));
+    const double bb = po_dot_f64(b, b, (size_t)n);
+    const double thr = relative ? tol * tol * bb : tol * tol;
+    double rr_prev = -1.0;
+    int total = 0, outer = 0;
+    for (outer = 0; outer <= max_outer; ++outer) {
+        if (outer == 0) {
+            memcpy(r, b, (size_t)n * sizeof(double)); /* x0 = 0 */
+        } else {
+            spmv(fmt, n, rowptr, col, val, ndiag, x, Ax);
+            for (int i = 0; i < n; ++i) r[i] = b[i] + (-1.0) * Ax[i];
+        }
+        const double rr = po_dot_f64(r, r, (size_t)n);
+        if (outer_hist && outer < hist_cap) outer_hist[outer] = rr;
+        if (rr < thr || outer == max_outer || rr == rr_prev) break;
+        rr_prev = rr;
+        for (int i = 0; i < n; ++i) { bf[i] = (float)r[i]; rf[i] = bf[i]; d[i] = 0.0f; }
+        double r1 = po_dot_f32(rf, rf, (size_t)n), r0 = 0.0;
+        const double ithr = inner_tol * inner_tol * po_dot_f32(bf, bf, (size_t)n);
+        int k = 1;
+        while (!(r1 < ithr) && k <= max_inner) {
+            if (k > 1) {
+                const float beta = (float)(r1 / r0);
+                for (int i = 0; i < n; ++i) p[i] = rf[i] + beta * p[i];
+            } else {
+                memcpy(p, rf, (size_t)n * sizeof(float));
+            }
+            spmv_f32(fmt, n, rowptr, col, valf, ndiag, p, Ap);
+            const double pAp = po_dot_f32(p, Ap, (size_t)n);
+            const float alpha = (float)(r1 / pAp);
+            const float na = -alpha;
+            for (int i = 0; i < n; ++i) d[i] = alpha * p[i] + d[i];
+            for (int i = 0; i < n; ++i) rf[i] = na * Ap[i] + rf[i];
+            r0 = r1;
+            r1 = po_dot_f32(rf, rf, (size_t)n);
+            ++k;
+        }
+        total += k - 1;
+        for (int i = 0; i < n; ++i) x[i] = x[i] + (double)d[i];
+    }
+    if (outer_out) *outer_out = outer;
+    free(valf); free(r); free(Ax); free(rf); free(bf); free(d); free(p); free(Ap);
+    return total;
+}

@@ -112,6 +112,19 @@ int po_cg_f64(int fmt, int n, const int32_t* rowptr, const int32_t* col, const d
               const double* b, double* x, double tol, int relative, int max_iter,
               double* hist, int hist_cap);
 
+/* FP32-inner / FP64-refinement CG (this build's mixed-precision form; the
+ * reference has none).  Outer step: r = b + (-1)*A x in FP64 (x0 = 0 skips
+ * the product), stop when r.r < tol^2*(relative ? b.b : 1), after max_outer
+ * steps, or when r.r repeats; else an FP32 CG (matrix values and r rounded
+ * to float, float SpMV/axpy, dot products of exact double products summed in
+ * double as po_dot_f32, alpha/beta = (float)(double ratio)) solves A d = r
+ * from d = 0 to r_f.r_f < inner_tol^2 * r.r, and x = x + (double)d.
+ * outer_hist[k] = FP64 r.r at outer step k.  Returns total inner iterations;
+ * *outer_out = refinement steps taken. */
+int po_cg_f32_refine(int fmt, int n, const int32_t* rowptr, const int32_t* col, const double* val, int ndiag,
+                     const double* b, double* x, double tol, int relative, double inner_tol, int max_inner,
+                     int max_outer, double* outer_hist, int hist_cap, int* outer_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,30 @@ class Grid2d(C.Structure):
                 ("steps", C.c_int32), ("result_in_b", C.c_int32)]
 
 
+class Csr(C.Structure):
+    _fields_ = [("n", C.c_int32), ("reserved0", C.c_int32), ("nnz", C.c_int64), ("rowptr", C.c_void_p),
+                ("col", C.c_void_p), ("val", C.c_void_p)]
+
+
+class Dia(C.Structure):
+    _fields_ = [("n", C.c_int32), ("ndiag", C.c_int32), ("offsets", C.c_void_p), ("val", C.c_void_p)]
+
+
+CG_F64, CG_F32_REFINE = 0, 1
+
+
+class CgOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("relative", C.c_int32), ("max_iter", C.c_int32),
+                ("precision", C.c_int32), ("max_inner", C.c_int32), ("max_outer", C.c_int32),
+                ("x0_is_zero", C.c_int32), ("blkpsm", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class CgReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32), ("final_rr", C.c_double),
+                ("bb", C.c_double), ("elapsed_ms", C.c_double), ("grid", C.c_int32), ("hist_cap", C.c_int32),
+                ("history", C.POINTER(C.c_double)), ("hist_len", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -112,6 +136,12 @@ SIGNATURES = {
     "perks_jacobi2d_iterative_f32": (_I, [_P, _I, _I, _P, _I, C.POINTER(StencilOpts), C.POINTER(RunReport)]),
     "perks_jacobi2d_device": (_I, [_I, _P, _P, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
     "perks_jacobi2d_batch": (_I, [_I, C.POINTER(Grid2d), _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
+    "perks_cg_opts_init": (None, [C.POINTER(CgOpts)]),
+    "perks_cg_solve_csr": (_I, [C.POINTER(Csr), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),
+    "perks_cg_solve_dia": (_I, [C.POINTER(Dia), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),
+    "perks_poisson7_nnz": (C.c_int64, [_I, _I, _I]),
+    "perks_poisson7_csr": (_I, [_I, _I, _I, _P, _P, _P, _P]),
+    "perks_poisson7_dia": (_I, [_I, _I, _I, _P, _P, _P]),
 }
 
 _lib = None

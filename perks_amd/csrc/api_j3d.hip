@@ -17,6 +17,7 @@
 #include "j3d_persistent.h"
 #include "planner.h"
 #include "j2d.h"
+#include "cg.h"
 
 namespace perks {
 
@@ -238,7 +239,8 @@ int perks_release(void)
 {
     const int a = perks::release_persistent_scratch();
     const int b = perks::release_j2d_scratch();
-    return a != PERKS_OK ? a : b;
+    const int c = perks::release_cg_scratch();
+    return a != PERKS_OK ? a : (b != PERKS_OK ? b : c);
 }
 
 int perks_j3d_iterative_f64(const double* h_in, int height, int width_y, int width_x, double* h_out, int steps,

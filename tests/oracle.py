@@ -68,6 +68,7 @@ def lib() -> C.CDLL:
             "po_dot_f64": (D, [P, P, S]),
             "po_dot_f32": (D, [P, P, S]),
             "po_cg_f64": (I, [I, I, P, P, P, I, P, P, D, I, I, P, I]),
+            "po_cg_f32_refine": (I, [I, I, P, P, P, I, P, P, D, I, D, I, I, P, I, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -251,3 +252,13 @@ def ref_j3d_gold_4step(field: np.ndarray) -> np.ndarray:
     f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     f(p(field), *field.shape, p(out))
     return out
+
+
+def cg_f32_refine(fmt, n, rowptr, col, val, ndiag, b, tol, relative, inner_tol, max_inner, max_outer):
+    x = np.zeros(n)
+    hist = np.zeros(max_outer + 2)
+    outer = C.c_int()
+    total = lib().po_cg_f32_refine(fmt, n, p(rowptr) if rowptr is not None else None, p(col), p(val), ndiag, p(b),
+                                   p(x), tol, int(relative), inner_tol, max_inner, max_outer, p(hist), hist.size,
+                                   C.byref(outer))
+    return x, hist[: outer.value + 1], total, outer.value
