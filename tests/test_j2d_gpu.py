@@ -119,3 +119,17 @@ def test_2d_reference_signature(gpu):
     assert rc == 0
     want = O.iterate(field, O.stencil_from_shape(P.shape_info("2d5pt"), "clamp"), 5)
     check_equal(out, want, "jacobi_iterative")
+
+
+GOLD = np.load(__import__("pathlib").Path(__file__).resolve().parent / "golden" / "ref_gold.npz")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+@pytest.mark.parametrize("method", ["plain", "persistent"])
+def test_2d_matches_committed_reference_gold(gpu, shape, tag, method):
+    field = GOLD[f"{shape}_{tag}_in"]
+    steps = int(GOLD["steps_2d"])
+    say(f"[j2d] reference gold vector {shape} {tag} {method} dims={field.shape} steps={steps}")
+    got, _ = P.jacobi2d_run(field, steps, shape=shape, method=method, boundary="clamp")
+    check_equal(got, GOLD[f"{shape}_{tag}_out"], f"{shape} {tag} {method} vs reference gold")

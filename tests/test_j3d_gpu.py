@@ -132,3 +132,26 @@ def test_persistent_result_buffer_parity(gpu):
         res = (b if rep.result_in_b else a).cpu().numpy().reshape(dims)
         want = O.iterate(O.uniform(n, 5, 0.0, 1.0).reshape(dims), O.stencil_from_shape(si, "clamp"), steps)
         assert np.array_equal(res, want), steps
+
+
+GOLD = np.load(__import__("pathlib").Path(__file__).resolve().parent / "golden" / "ref_gold.npz")
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+@pytest.mark.parametrize("method", ["plain", "persistent"])
+def test_matches_committed_reference_gold(gpu, tag, method):
+    field = GOLD[f"3d7pt_{tag}_in"]
+    steps = int(GOLD["steps_3d"])
+    say(f"[j3d] reference gold vector 3d7pt {tag} {method} dims={field.shape} steps={steps}")
+    got, _ = P.j3d_run(field, steps, method=method, boundary="clamp")
+    assert np.array_equal(got.view(np.uint8), GOLD[f"3d7pt_{tag}_out"].view(np.uint8))
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+@pytest.mark.parametrize("method", ["plain", "persistent"])
+def test_matches_committed_interior_gold(gpu, tag, method):
+    """The reference's interior-only j3d_gold: zero-halo form on the interior."""
+    f = GOLD[f"j3d_gold4_{tag}_in"]
+    say(f"[j3d] interior gold 3d7pt {tag} {method}")
+    got, _ = P.j3d_run(np.ascontiguousarray(f[1:-1, 1:-1, 1:-1]), 4, method=method, boundary="zero")
+    assert np.array_equal(got, GOLD[f"j3d_gold4_{tag}_out"][1:-1, 1:-1, 1:-1])

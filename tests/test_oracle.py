@@ -7,9 +7,13 @@
   symmetric star stencils with zero halo; a constant field under clamp);
 * against an independent numpy restatement (padding instead of index clamps)
   on small grids, bit for bit;
-* the counter RNG against SplitMix64's published first outputs.
+* against the same gold's outputs committed as fixtures (tests/golden), which
+  hold where oracle/_ref cannot be built;
+* the counter RNG against SplitMix64's published first outputs;
+* the FP16/BF16 rounding against numpy's float16 and torch's bfloat16.
 """
 import math
+import pathlib
 
 import numpy as np
 import pytest
@@ -201,3 +205,100 @@ def test_rms_error_metric():
     rms, mx = O.rms_error(a, b)
     assert mx == 3.0
     assert abs(rms - math.sqrt(9.0 / 24)) < 1e-15
+
+
+# ---------------------------------------------------------------- 16-bit storage formats
+def _f32_samples():
+    rng = np.random.default_rng(17)
+    bits = rng.integers(0, 2 ** 32, 60000, dtype=np.uint64).astype(np.uint32)
+    # exact ties and neighbours of ties for binary16 (13 dropped bits) and bfloat16 (16 dropped bits),
+    # the binary16 subnormal range, the overflow threshold and the specials
+    base = rng.integers(0, 2 ** 32, 4000, dtype=np.uint64).astype(np.uint32)
+    ties = np.concatenate([(base & ~np.uint32(0x1FFF)) | np.uint32(0x1000), (base & ~np.uint32(0xFFFF)) | np.uint32(0x8000)])
+    ties = np.concatenate([ties, ties + np.uint32(1), ties - np.uint32(1)])
+    sub = np.linspace(0, 2.0 ** -14, 5001, dtype=np.float32).view(np.uint32)
+    special = np.array([0x00000000, 0x80000000, 0x7F800000, 0xFF800000, 0x477FE000, 0x477FEFFF, 0x477FF000,
+                        0x477FF001, 0x33000000, 0x33000001, 0x32FFFFFF, 0x38800000, 0x387FFFFF, 0x7F7FFFFF],
+                       dtype=np.uint32)
+    return np.concatenate([bits, ties, sub, sub | np.uint32(0x80000000), special]).view(np.float32)
+
+
+def _oracle_round(x, fmt):
+    f = O.lib().po_f32_to_f16 if fmt == "f16" else O.lib().po_f32_to_bf16
+    return np.array([f(float(v)) for v in x], dtype=np.uint16)
+
+
+def test_f16_rounding_matches_numpy():
+    x = _f32_samples()
+    finite = ~np.isnan(x)
+    got = _oracle_round(x[finite], "f16")
+    with np.errstate(over="ignore"):
+        want = x[finite].astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+    nan_bits = _oracle_round(x[np.isnan(x)], "f16")
+    assert nan_bits.size > 0 and np.all(np.isnan(nan_bits.view(np.float16)))
+
+
+def test_f16_widening_is_exact_for_every_pattern():
+    h = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    got = np.array([O.lib().po_f16_to_f32(int(v)) for v in h], dtype=np.float32)
+    want = h.view(np.float16).astype(np.float32)
+    ok = ~np.isnan(want)
+    assert np.array_equal(got[ok].view(np.uint32), want[ok].view(np.uint32))
+    assert np.all(np.isnan(got[~ok]))
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    x = _f32_samples()
+    finite = ~np.isnan(x)
+    got = _oracle_round(x[finite], "bf16")
+    want = torch.from_numpy(x[finite].copy()).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(got, want)
+    back = np.array([O.lib().po_bf16_to_f32(int(v)) for v in got[:2000]], dtype=np.float32)
+    assert np.array_equal(back, torch.from_numpy(got[:2000].view(np.int16).copy()).view(torch.bfloat16).float().numpy())
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_reduced_storage_iteration_matches_numpy_restatement(fmt):
+    """Each step in float32 on the widened values, then rounded to the storage
+    format (numpy float16 / torch bfloat16 do the rounding here)."""
+    torch = pytest.importorskip("torch")
+    si = P.shape_info("2d5pt")
+    coefs = [0.15, 0.2, 0.3, 0.2, 0.15]
+    groups = [(coefs[g], [(0,) + tuple(o[1:]) for o in si.offsets[g]], 0) for g in range(5)]
+    u = O.uniform(23 * 37, 9, -1.0, 1.0, np.float32).reshape(1, 23, 37)
+    bits = O.to16(u, fmt)
+
+    def rnd(a):
+        if fmt == "f16":
+            return a.astype(np.float16).astype(np.float32)
+        return torch.from_numpy(a.copy()).to(torch.bfloat16).float().numpy()
+
+    want = O.from16(bits, fmt)
+    for _ in range(6):
+        want = rnd(numpy_step(want, groups, "zero"))
+    got = O.from16(O.iterate16(bits.reshape(23, 37), O.stencil(groups, "zero"), 6, fmt), fmt)
+    assert np.array_equal(got.reshape(23, 37), want.reshape(23, 37))
+
+
+# ---------------------------------------------------------------- committed reference golden vectors
+GOLD = np.load(pathlib.Path(__file__).resolve().parent / "golden" / "ref_gold.npz")
+
+
+@pytest.mark.parametrize("shape", sorted(REF_3D) + sorted(REF_2D))
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+def test_oracle_matches_committed_reference_vectors(shape, tag):
+    """Outputs of the reference's gold on glibc-rand() inputs, written by
+    tests/golden/make_golden.py; holds without oracle/_ref."""
+    field = GOLD[f"{shape}_{tag}_in"]
+    steps = int(GOLD["steps_3d"] if field.ndim == 3 else GOLD["steps_2d"])
+    got = O.iterate(field, O.stencil_from_shape(P.shape_info(shape), "clamp"), steps)
+    assert np.array_equal(got.view(np.uint8), GOLD[f"{shape}_{tag}_out"].view(np.uint8))
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+def test_oracle_matches_committed_interior_gold(tag):
+    f = GOLD[f"j3d_gold4_{tag}_in"]
+    got = O.iterate(np.ascontiguousarray(f[1:-1, 1:-1, 1:-1]), O.stencil_from_shape(P.shape_info("3d7pt"), "zero"), 4)
+    assert np.array_equal(got, GOLD[f"j3d_gold4_{tag}_out"][1:-1, 1:-1, 1:-1])
