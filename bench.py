@@ -13,6 +13,16 @@ Prints one JSON line (see the keys at the bottom).  GCells/s counts
 time steps * cells / seconds, the reference's own definition
 (stencil/3dstencil/j3d.cu:754); the roofline prices one read plus one write
 of the field per cell-update (j3d.cu:756) against 8 TB/s of HBM.
+
+Other workloads (--workload; each prints its own JSON line):
+  j2d-storage  2D box r1 (2d9pt) and r2 (2d25pt), 8192^2, 500 steps, storage
+               FP32 / FP16 / BF16 with FP32 arithmetic, against FP64 on the
+               same inputs (max and RMS error at 10 / 100 / 500 steps).
+  batch2d      512 independent 2D grids (sides log-uniform in [64,1024],
+               steps uniform in [50,500], FP32) in one persistent launch,
+               against solving them one after another with the per-step kernel.
+  cg           CG on the 7-point Poisson matrix, 256^3, CSR and DIA, FP64 and
+               FP32-inner/FP64-refinement, to ||r||/||b|| < 1e-10.
 """
 from __future__ import annotations
 
@@ -44,6 +54,8 @@ def parse():
     ap.add_argument("--no-per-step", action="store_true", help="skip the per-step comparison")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
+    ap.add_argument("--workload", choices=["j3d", "j2d-storage", "batch2d", "cg"], default="j3d")
+    ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
 
@@ -101,19 +113,10 @@ def load_traffic(n, tsteps, dtype):
     return d.get("hbm_bytes_per_launch")
 
 
-def main():
-    args = parse()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the persistent stencil path is single-device "
-              "(run with --gpus 1)", file=sys.stderr)
-        sys.exit(2)
-
+def run_j3d(args):
     import torch
     import perks_amd as P
 
-    if not torch.cuda.is_available():
-        print("bench.py needs a GPU", file=sys.stderr)
-        sys.exit(1)
     n, T = args.size, args.tsteps
     cells = n ** 3
     tdt = torch.float64 if args.dtype == "f64" else torch.float32
@@ -184,6 +187,264 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps, args.dtype)
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------- 2D storage precision
+def _errors(torch, got, ref):
+    """max and RMS of |got - ref|, absolute and relative to max|ref| / RMS(ref)."""
+    d = (got - ref).abs()
+    mx, rms = d.max().item(), d.square().mean().sqrt().item()
+    rmx, rrms = ref.abs().max().item(), ref.square().mean().sqrt().item()
+    return {"max_abs": mx, "rms_abs": rms, "max_rel": mx / rmx if rmx else None,
+            "rms_rel": rms / rrms if rrms else None}
+
+
+def run_j2d_storage(args):
+    import numpy as np
+    import torch
+    import perks_amd as P
+
+    n = 8192 if args.size == 512 else args.size
+    T = 500 if args.tsteps == 1000 else args.tsteps
+    checkpoints = [c for c in (10, 100, 500) if c < T] + [T]
+    cells = n * n
+    storage = {"f64": torch.float64, "f32": torch.float32, "f16": torch.int16, "bf16": torch.int16}
+    view = {"f16": torch.float16, "bf16": torch.bfloat16}
+    results, kernel_ms = {}, {}
+    for shape in ("2d9pt", "2d25pt"):
+        si = P.shape_info(shape)
+        c = np.random.default_rng(args.seed).random(len(si.coef))
+        coefs = list(c / c.sum())
+        ref_at = {}
+        per = {}
+        for dt in ("f64", "f32", "f16", "bf16"):
+            a = torch.empty(cells, dtype=storage[dt], device="cuda")
+            b = torch.empty_like(a)
+            init = torch.empty_like(a)
+            P.fill_uniform(init, cells, args.seed, -1.0, 1.0, dtype=dt)
+            opts = P.make_opts(shape, "clamp", "persistent", coefs)
+            row = {}
+            for steps in checkpoints:
+                a.copy_(init)
+                torch.cuda.synchronize()
+                rep = P.jacobi2d_device(a, b, n, n, steps, dtype=dt, opts=opts)
+                res = b if rep.result_in_b else a
+                val = res.view(view[dt]).double() if dt in view else res.double()
+                if dt == "f64":
+                    ref_at[steps] = val.clone()
+                else:
+                    row[str(steps)] = _errors(torch, val, ref_at[steps])
+                if steps == T:
+                    times = []
+                    for _ in range(max(1, args.steps)):
+                        a.copy_(init)
+                        torch.cuda.synchronize()
+                        times.append(P.jacobi2d_device(a, b, n, n, T, dtype=dt, opts=opts).elapsed_ms)
+                    ms = sum(times) / len(times)
+                    kernel_ms[(shape, dt)] = ms
+                    row["gcells_per_s"] = round(T * cells / (ms / 1e3) / 1e9, 2)
+                    row["ms_per_solve"] = round(ms, 3)
+                    row["steps_per_pass"] = rep.reserved[0]
+                    plain = P.make_opts(shape, "clamp", "plain", coefs)
+                    a.copy_(init)
+                    torch.cuda.synchronize()
+                    pr = P.jacobi2d_device(a, b, n, n, T, dtype=dt, opts=plain)
+                    row["per_step_kernel_gcells_per_s"] = round(T * cells / (pr.elapsed_ms / 1e3) / 1e9, 2)
+            per[dt] = row
+            del a, b, init
+        del ref_at
+        torch.cuda.empty_cache()
+        for dt in ("f32", "f16", "bf16"):
+            per[dt]["speedup_vs_f32"] = round(per[dt]["gcells_per_s"] / per["f32"]["gcells_per_s"], 3)
+        results[shape] = per
+    bf = results["2d9pt"]["bf16"]
+    ms = kernel_ms[("2d9pt", "bf16")]
+    alg = T * cells * 2 * 2
+    achieved = alg / (ms / 1e3) / 1e9
+    out = {
+        "metric": "GCells/s, 2D box r1/r2 8192^2 500 steps, FP32 vs FP16 vs BF16 storage (FP32 arithmetic), "
+                  "error vs FP64",
+        "value": bf["gcells_per_s"], "unit": "GCells/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 storage / f32 arithmetic", "data": "synthetic",
+        "config": {"workload": f"2D box stencils 2d9pt (r1) and 2d25pt (r2), {n}^2, clamp boundary, coefficients "
+                               f"uniform[0,1) normalised to sum 1 (seed {args.seed}), input uniform[-1,1), {T} steps, "
+                               "persistent kernel; value = 2d9pt BF16", "size": n, "time_steps": T,
+                   "results": results},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "j2d_kernel<2d9pt, bf16s>", "algorithmic_bytes_per_launch": alg},
+    }
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------- batched 2D grids
+def batch_specs(seed, count=512):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = np.exp(rng.uniform(np.log(64), np.log(1024), size=(count, 2))).astype(int)
+    steps = rng.integers(50, 501, size=count)
+    return [(int(h), int(w), int(t)) for (h, w), t in zip(sides, steps)]
+
+
+def run_batch2d(args):
+    import torch
+    import perks_amd as P
+
+    specs = batch_specs(args.seed)
+    cell_steps = sum(h * w * t for h, w, t in specs)
+    out_rows = {}
+    for shape in ("2d5pt", "2d9pt"):
+        si = P.shape_info(shape)
+        coefs = [1.0 / len(si.coef)] * len(si.coef)
+        bufs = []
+        for i, (h, w, _) in enumerate(specs):
+            a = torch.empty(h * w, dtype=torch.float32, device="cuda")
+            P.fill_uniform(a, h * w, args.seed * 100003 + i, 0.0, 1.0, dtype="f32")
+            bufs.append((a.clone(), a, torch.empty_like(a)))
+        grids = [(a, b, h, w, t) for (_, a, b), (h, w, t) in zip(bufs, specs)]
+
+        def reset():
+            for init, a, _ in bufs:
+                a.copy_(init)
+            torch.cuda.synchronize()
+
+        opts = P.make_opts(shape, "zero", "persistent", coefs)
+        reset()
+        P.jacobi2d_batch(grids, dtype="f32", opts=opts)
+        times = []
+        for _ in range(max(1, args.steps)):
+            reset()
+            rep, flags = P.jacobi2d_batch(grids, dtype="f32", opts=opts)
+            times.append(rep.elapsed_ms)
+        batched = [(b if f else a).clone() for (_, a, b), f in zip(bufs, flags)]
+        ms = sum(times) / len(times)
+        # One grid after another with the per-step kernel (one launch per time step).
+        plain = P.make_opts(shape, "zero", "plain", coefs)
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        seq = []
+        for (init, a, b), (h, w, t) in zip(bufs, specs):
+            r = P.jacobi2d_device(a, b, h, w, t, dtype="f32", opts=plain)
+            seq.append(b if r.result_in_b else a)
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t0) * 1e3
+        equal = all(torch.equal(x.view(torch.int32), y.view(torch.int32)) for x, y in zip(batched, seq))
+        out_rows[shape] = {"ms_per_batch": round(ms, 3), "gcells_per_s": round(cell_steps / (ms / 1e3) / 1e9, 2),
+                           "sequential_per_step_ms": round(seq_ms, 3),
+                           "sequential_per_step_gcells_per_s": round(cell_steps / (seq_ms / 1e3) / 1e9, 2),
+                           "speedup": round(seq_ms / ms, 3), "bitwise_equal": equal,
+                           "grid": rep.grid[0], "steps_per_pass": rep.reserved[0], "passes": rep.reserved[1]}
+        del bufs, grids, batched, seq
+        torch.cuda.empty_cache()
+    r = out_rows["2d5pt"]
+    alg = cell_steps * 2 * 4
+    achieved = alg / (r["ms_per_batch"] / 1e3) / 1e9
+    out = {
+        "metric": "GCells/s, 512 batched 2D grids (5pt/9pt FP32) in one persistent launch vs sequential per-step",
+        "value": r["gcells_per_s"], "unit": "GCells/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": r["ms_per_batch"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"512 grids, sides log-uniform in [64,1024] (non-square), steps uniform in [50,500], "
+                               f"FP32, zero boundary, equal weights, seed {args.seed}; value = 2d5pt",
+                   "grids": len(specs), "cell_steps": cell_steps, "results": out_rows},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "j2d_kernel<2d5pt, float> (batched)", "algorithmic_bytes_per_launch": alg},
+    }
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------- CG
+def cg_bytes_per_iter(fmt, n, nnz, vbytes):
+    """Bytes one iteration moves in the reference's model (cg_driver.cu:359-361):
+    the matrix once plus six vector streams."""
+    if fmt == "csr":
+        return nnz * vbytes + (nnz + n + 1) * 4 + 6 * n * vbytes
+    return 7 * n * vbytes + 6 * n * vbytes
+
+
+def run_cg(args):
+    import torch
+    import perks_amd as P
+
+    m = 256 if args.size == 512 else args.size
+    n = m ** 3
+    nnz = P.poisson7_nnz(m, m, m)
+    b = torch.empty(n, dtype=torch.float64, device="cuda")
+    P.fill_uniform(b, n, args.seed, -1.0, 1.0, dtype="f64")
+    rows = {}
+    for fmt in ("csr", "dia"):
+        if fmt == "csr":
+            A = dict(rowptr=torch.empty(n + 1, dtype=torch.int32, device="cuda"),
+                     col=torch.empty(nnz, dtype=torch.int32, device="cuda"),
+                     val=torch.empty(nnz, dtype=torch.float64, device="cuda"), nnz=nnz)
+            P.poisson7_csr(m, m, m, A["rowptr"], A["col"], A["val"])
+        else:
+            val = torch.empty(7 * n, dtype=torch.float64, device="cuda")
+            A = dict(val=val, offsets=P.poisson7_dia(m, m, m, val))
+        for prec in ("f64", "f32_refine"):
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            P.cg_solve(fmt, n, b, x, tol=1e-10, precision=prec, **A)  # warm-up
+            times = []
+            for _ in range(max(1, args.steps)):
+                x.zero_()
+                torch.cuda.synchronize()
+                res = P.cg_solve(fmt, n, b, x, tol=1e-10, precision=prec, **A)
+                times.append(res.elapsed_ms)
+            ms = sum(times) / len(times)
+            # independent FP64 residual with torch on the 3D grid view
+            xv = x.view(m, m, m)
+            ax = 6.0 * xv
+            ax[1:] -= xv[:-1]
+            ax[:-1] -= xv[1:]
+            ax[:, 1:] -= xv[:, :-1]
+            ax[:, :-1] -= xv[:, 1:]
+            ax[:, :, 1:] -= xv[:, :, :-1]
+            ax[:, :, :-1] -= xv[:, :, 1:]
+            true_rel = ((b - ax.reshape(-1)).norm() / b.norm()).item()
+            vb = 8 if prec == "f64" else 4
+            per_iter = cg_bytes_per_iter(fmt, n, nnz, vb)
+            rows[f"{fmt}_{prec}"] = {
+                "code": res.code, "iterations": res.iterations, "outer_iterations": res.outer_iterations,
+                "time_to_tolerance_ms": round(ms, 3), "ms_per_iteration": round(ms / max(1, res.iterations), 4),
+                "final_rel_residual_fp64": res.relative_residual, "true_rel_residual_fp64": true_rel,
+                "grid": res.grid,
+                "achieved_gbs": round(per_iter * res.iterations / (ms / 1e3) / 1e9, 1),
+            }
+        del A
+        torch.cuda.empty_cache()
+    r = rows["csr_f64"]
+    achieved = r["achieved_gbs"]
+    out = {
+        "metric": "CG time-to-tolerance, 7pt Poisson 256^3, CSR/DIA, FP64 vs FP32-inner/FP64-refinement",
+        "value": r["time_to_tolerance_ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": r["time_to_tolerance_ms"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"CG on the 3D Poisson 7-point matrix (diag 6, off-diag -1), n={m}^3, rhs "
+                               f"uniform[-1,1) (seed {args.seed}), x0=0, ||r||/||b|| < 1e-10; value = CSR FP64",
+                   "n": n, "nnz": nnz, "results": rows},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "cg_persistent_kernel<double, CSR>",
+                     "algorithmic_bytes_per_launch": cg_bytes_per_iter("csr", n, nnz, 8) * r["iterations"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the persistent stencil path is single-device "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    {"j3d": run_j3d, "j2d-storage": run_j2d_storage, "batch2d": run_batch2d, "cg": run_cg}[args.workload](args)
 
 
 if __name__ == "__main__":

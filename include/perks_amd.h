@@ -45,6 +45,7 @@ extern "C" {
 #define PERKS_ERR_NO_DEVICE (-12)
 #define PERKS_ERR_UNSUPPORTED (-13)     /* shape/dtype/method combination not built     */
 #define PERKS_ERR_NOT_CONVERGED (-14)
+#define PERKS_ERR_BARRIER_TIMEOUT (-15) /* a grid barrier wait exceeded 10 s; launch abandoned */
 
 /* ---- enums ---------------------------------------------------------------- */
 enum perks_dtype { PERKS_F64 = 0, PERKS_F32 = 1, PERKS_F16 = 2, PERKS_BF16 = 3 };

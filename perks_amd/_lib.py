@@ -22,6 +22,7 @@ ERR_HIP = -11
 ERR_NO_DEVICE = -12
 ERR_UNSUPPORTED = -13
 ERR_NOT_CONVERGED = -14
+ERR_BARRIER_TIMEOUT = -15
 
 F64, F32, F16, BF16 = 0, 1, 2, 3
 BC_CLAMP, BC_ZERO = 0, 1

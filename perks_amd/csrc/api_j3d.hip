@@ -176,6 +176,7 @@ const char* perks_error_string(int code)
     case PERKS_ERR_NO_DEVICE: return "no HIP device";
     case PERKS_ERR_UNSUPPORTED: return "shape/dtype/method combination not built";
     case PERKS_ERR_NOT_CONVERGED: return "solver did not converge";
+    case PERKS_ERR_BARRIER_TIMEOUT: return "grid barrier timed out (launch abandoned)";
     default: return "unknown error";
     }
 }

@@ -32,6 +32,7 @@
 
 #include "api_common.h"
 #include "cg.h"
+#include "grid_sync.h"
 
 namespace perks {
 
@@ -40,8 +41,13 @@ namespace {
 constexpr int kLanes = 256;
 constexpr int kPerLane = 16;
 constexpr int kSeg = kLanes * kPerLane;
+constexpr int kMaxDiag = 16;
+constexpr int kCsrChunk = 8;  // row entries loaded per predicated chunk
+constexpr int kMaxWgPerCu = 4;
 
-enum { FMT_CSR = 0, FMT_DIA = 1 };
+// Matrix formats the kernel is built for.  DIA7 is the 7-diagonal form with
+// the diagonal count fixed at compile time (every load of a row issued at once).
+enum { FMT_CSR = 0, FMT_DIA = 1, FMT_DIA7 = 2 };
 
 template <typename V>
 struct CgArgs {
@@ -52,7 +58,7 @@ struct CgArgs {
     const int32_t* col;
     const V* val;
     int ndiag;
-    int32_t offs[16];
+    int32_t offs[kMaxDiag];
     // vectors
     V* x;
     V* r;
@@ -72,23 +78,10 @@ struct CgArgs {
     int* iters_out;
     double* rr_out;
     double* bb_out;
-    unsigned int* barrier;
+    GridSync* sync;
+    unsigned long long timeout;
     int x_is_zero;      // skip the initial SpMV
 };
-
-__device__ __forceinline__ void cg_barrier(unsigned int* counter, unsigned int nblocks, unsigned int epoch)
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned int target = epoch * nblocks;
-        while ((int)(__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0)
-            __builtin_amdgcn_s_sleep(2);
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
 
 // Folds the 256 per-lane values of a workgroup pairwise with strides
 // 128, 64, ..., 1 (v[l] += v[l + s]); returns the result in every lane.
@@ -112,7 +105,8 @@ __device__ __forceinline__ double fold256(double v, double* lds)
     return r;
 }
 
-// Sum of the segment partials in the fixed order (see header).
+// Sum of the segment partials in the fixed order: lane l sums partials
+// l, l+256, ... in order, then the 256-lane fold.
 __device__ __forceinline__ double reduce_parts(const double* part, int nseg, double* lds)
 {
     double acc = 0.0;
@@ -120,42 +114,50 @@ __device__ __forceinline__ double reduce_parts(const double* part, int nseg, dou
     return fold256(acc, lds);
 }
 
-template <typename V, int FMT>
-__device__ __forceinline__ V spmv_row(const CgArgs<V>& a, int i, const V* vec)
+// Vector entry j of the operand: r_j for the first iteration (p_1 = r_0),
+// else p_j = r_j + beta * pold_j formed on the fly.
+template <typename V, bool FIRST>
+__device__ __forceinline__ V operand(const V* __restrict__ r, const V* __restrict__ pold, V beta, int j)
 {
-    V s = (V)0;
-    if (FMT == FMT_CSR) {
-        const int k0 = a.rowptr[i], k1 = a.rowptr[i + 1];
-        for (int k = k0; k < k1; ++k) s = s + a.val[k] * vec[a.col[k]];
-    } else {
-        for (int d = 0; d < a.ndiag; ++d) {
-            const long long j = (long long)i + a.offs[d];
-            if (j < 0 || j >= a.n) continue;
-            s = s + a.val[(size_t)d * a.n + i] * vec[j];
-        }
-    }
-    return s;
+    return FIRST ? r[j] : r[j] + beta * pold[j];
 }
 
-// Row i of A applied to p_k = r + beta * p_{k-1}, formed on the fly from the
-// previous p (first iteration: p_1 = r).
-template <typename V, int FMT>
-__device__ __forceinline__ V spmv_row_fused(const CgArgs<V>& a, int i, const V* r, const V* pold, V beta, bool first)
+// Row i of A times the operand.  Entries are summed in storage order from
+// zero; DIA entries whose column falls outside [0, n) are skipped.  All loads
+// are unconditional (out-of-range columns read row i itself and the sum keeps
+// its value through a select), so a row's loads are in flight together.
+template <typename V, int FMT, bool FIRST>
+__device__ __forceinline__ V row_product(const CgArgs<V>& a, int i, const V* __restrict__ r,
+                                         const V* __restrict__ pold, V beta)
 {
     V s = (V)0;
     if (FMT == FMT_CSR) {
         const int k0 = a.rowptr[i], k1 = a.rowptr[i + 1];
-        for (int k = k0; k < k1; ++k) {
-            const int j = a.col[k];
-            const V pj = first ? r[j] : r[j] + beta * pold[j];
-            s = s + a.val[k] * pj;
+        for (int kb = k0; kb < k1; kb += kCsrChunk) {
+            int cj[kCsrChunk];
+            V cv[kCsrChunk];
+#pragma unroll
+            for (int u = 0; u < kCsrChunk; ++u) {
+                const int k = (kb + u < k1) ? kb + u : k0;
+                cj[u] = a.col[k];
+                cv[u] = a.val[k];
+            }
+#pragma unroll
+            for (int u = 0; u < kCsrChunk; ++u) {
+                const V prod = cv[u] * operand<V, FIRST>(r, pold, beta, cj[u]);
+                s = (kb + u < k1) ? s + prod : s;
+            }
         }
     } else {
-        for (int d = 0; d < a.ndiag; ++d) {
+        const int nd = FMT == FMT_DIA7 ? 7 : a.ndiag;
+#pragma unroll
+        for (int d = 0; d < (FMT == FMT_DIA7 ? 7 : kMaxDiag); ++d) {
+            if (FMT != FMT_DIA7 && d >= nd) break;
             const long long j = (long long)i + a.offs[d];
-            if (j < 0 || j >= a.n) continue;
-            const V pj = first ? r[j] : r[j] + beta * pold[j];
-            s = s + a.val[(size_t)d * a.n + i] * pj;
+            const bool in = (unsigned long long)j < (unsigned long long)a.n;
+            const int jc = in ? (int)j : i;
+            const V prod = a.val[(size_t)d * a.n + i] * operand<V, FIRST>(r, pold, beta, jc);
+            s = in ? s + prod : s;
         }
     }
     return s;
@@ -167,6 +169,30 @@ __device__ __forceinline__ double prod_d(V a, V b)
     return (double)a * (double)b;  // exact for float operands; the product itself for double
 }
 
+// Phase 1 of an iteration over this workgroup's segments: p = r + beta*p
+// (stored into pnew), Ap = A p, partial sums of p.Ap into part_a.
+template <typename V, int FMT, bool FIRST>
+__device__ __forceinline__ void phase_spmv(const CgArgs<V>& a, const V* pold, V* pnew, V beta, double* lds)
+{
+    const int t = threadIdx.x;
+    for (int s = blockIdx.x; s < a.nseg; s += gridDim.x) {
+        double acc = 0.0;
+#pragma unroll 2
+        for (int kk = 0; kk < kPerLane; ++kk) {
+            const int i = s * kSeg + kk * kLanes + t;
+            if (i < a.n) {
+                const V pi = operand<V, FIRST>(a.r, pold, beta, i);
+                const V api = row_product<V, FMT, FIRST>(a, i, a.r, pold, beta);
+                pnew[i] = pi;
+                a.ap[i] = api;
+                acc = acc + prod_d(pi, api);
+            }
+        }
+        const double ps = fold256(acc, lds);
+        if (t == 0) a.part_a[s] = ps;
+    }
+}
+
 template <typename V, int FMT>
 __global__ void __launch_bounds__(kLanes) cg_persistent_kernel(CgArgs<V> a)
 {
@@ -174,17 +200,16 @@ __global__ void __launch_bounds__(kLanes) cg_persistent_kernel(CgArgs<V> a)
     const int t = threadIdx.x;
     const int G = gridDim.x;
     const int me = blockIdx.x;
-    unsigned int epoch = 0;
 
     // ---- r = b + (-1) * A x ; partials of r.r and b.b ----
     for (int s = me; s < a.nseg; s += G) {
         double rr = 0.0, bb = 0.0;
-#pragma unroll 4
+#pragma unroll 2
         for (int k = 0; k < kPerLane; ++k) {
             const int i = s * kSeg + k * kLanes + t;
             if (i < a.n) {
                 const V bi = a.b[i];
-                const V ri = a.x_is_zero ? bi : bi + (V)(-1) * spmv_row<V, FMT>(a, i, a.x);
+                const V ri = a.x_is_zero ? bi : bi + (V)(-1) * row_product<V, FMT, true>(a, i, a.x, a.x, (V)0);
                 a.r[i] = ri;
                 rr = rr + prod_d(ri, ri);
                 bb = bb + prod_d(bi, bi);
@@ -197,7 +222,7 @@ __global__ void __launch_bounds__(kLanes) cg_persistent_kernel(CgArgs<V> a)
             a.part_b[s] = pbb;
         }
     }
-    cg_barrier(a.barrier, G, ++epoch);
+    if (!grid_sync(a.sync, G, a.timeout)) return;
     double r1 = reduce_parts(a.part_a, a.nseg, lds);
     const double bbv = reduce_parts(a.part_b, a.nseg, lds);
     const double thr = a.relative ? a.tol2 * bbv : a.tol2;
@@ -205,33 +230,20 @@ __global__ void __launch_bounds__(kLanes) cg_persistent_kernel(CgArgs<V> a)
         if (a.hist && a.hist_cap > 0) a.hist[0] = r1;
         *a.bb_out = bbv;
     }
-    cg_barrier(a.barrier, G, ++epoch);  // part_a/part_b are rewritten below
+    if (!grid_sync(a.sync, G, a.timeout)) return;  // part_a/part_b are rewritten below
 
     double r0 = 0.0;
     int k = 1;
     V* pold = a.p0;
     V* pnew = a.p1;
     while (!(r1 < thr) && k <= a.max_iter) {
-        const bool first = k == 1;
-        const V beta = first ? (V)0 : (V)(r1 / r0);
         // ---- p = r + beta*p ; Ap = A p ; partials of p.Ap ----
-        for (int s = me; s < a.nseg; s += G) {
-            double acc = 0.0;
-#pragma unroll 2
-            for (int kk = 0; kk < kPerLane; ++kk) {
-                const int i = s * kSeg + kk * kLanes + t;
-                if (i < a.n) {
-                    const V pi = first ? a.r[i] : a.r[i] + beta * pold[i];
-                    const V api = spmv_row_fused<V, FMT>(a, i, a.r, pold, beta, first);
-                    pnew[i] = pi;
-                    a.ap[i] = api;
-                    acc = acc + prod_d(pi, api);
-                }
-            }
-            const double ps = fold256(acc, lds);
-            if (t == 0) a.part_a[s] = ps;
+        if (k == 1) {
+            phase_spmv<V, FMT, true>(a, pold, pnew, (V)0, lds);
+        } else {
+            phase_spmv<V, FMT, false>(a, pold, pnew, (V)(r1 / r0), lds);
         }
-        cg_barrier(a.barrier, G, ++epoch);
+        if (!grid_sync(a.sync, G, a.timeout)) return;
         const double pap = reduce_parts(a.part_a, a.nseg, lds);
         const double alpha_d = r1 / pap;
         const V alpha = (V)alpha_d;
@@ -252,7 +264,7 @@ __global__ void __launch_bounds__(kLanes) cg_persistent_kernel(CgArgs<V> a)
             const double ps = fold256(acc, lds);
             if (t == 0) a.part_b[s] = ps;
         }
-        cg_barrier(a.barrier, G, ++epoch);
+        if (!grid_sync(a.sync, G, a.timeout)) return;
         r0 = r1;
         r1 = reduce_parts(a.part_b, a.nseg, lds);
         if (me == 0 && t == 0 && a.hist && k < a.hist_cap) a.hist[k] = r1;
@@ -283,11 +295,6 @@ __global__ void zero_f32(float* __restrict__ d, int n)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = 0.0f;
 }
 
-__global__ void zero_u32_cg(unsigned int* p, int n)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
-}
-
 // ---- scratch ---------------------------------------------------------------
 struct CgScratch {
     size_t n = 0;
@@ -301,7 +308,6 @@ struct CgScratch {
     float* valf = nullptr; // FP32 copy of the matrix values (refinement)
     size_t nvalf = 0;
     double* parts = nullptr;
-    unsigned int* barrier = nullptr;
     double* small = nullptr;  // [0] rr, [1] bb ; int at [2]
     double* hist = nullptr;
     int hist_cap = 0;
@@ -330,7 +336,6 @@ int ensure(CgScratch& s, size_t n, size_t nval32, int hist_cap)
         PERKS_HIP_TRY(hipMalloc(&s.valf, nval32 * 4));
         s.nvalf = nval32;
     }
-    if (!s.barrier) PERKS_HIP_TRY(hipMalloc(&s.barrier, 256));
     if (!s.small) PERKS_HIP_TRY(hipMalloc(&s.small, 64));
     if (s.hist_cap < hist_cap) {
         if (s.hist) PERKS_HIP_TRY(hipFree(s.hist));
@@ -348,13 +353,17 @@ int launch_cg(CgArgs<V>& a, hipStream_t stream, int blkpsm, int* grid_out)
     PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kLanes, 0));
     if (occ <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
     if (blkpsm > 0 && blkpsm < occ) occ = blkpsm;
+    // At most half the wave slots: 4 workgroups of 4 waves per CU keep enough
+    // loads in flight, and the cooperative grid never needs every slot free.
+    if (occ > kMaxWgPerCu) occ = kMaxWgPerCu;
     perks_device_caps caps;
     int rc = device_caps_cached(&caps);
     if (rc != PERKS_OK) return rc;
     int G = caps.cu_count * occ;
     if (G > a.nseg) G = a.nseg > 0 ? a.nseg : 1;
-    hipLaunchKernelGGL(zero_u32_cg, dim3(1), dim3(64), 0, stream, a.barrier, 64);
-    PERKS_HIP_TRY(hipGetLastError());
+    rc = grid_sync_get(kSyncCg, &a.sync);
+    if (rc != PERKS_OK) return rc;
+    a.timeout = grid_sync_timeout();
     void* args[] = {&a};
     PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(G), dim3(kLanes), args, 0, stream));
     if (grid_out) *grid_out = G;
@@ -364,8 +373,9 @@ int launch_cg(CgArgs<V>& a, hipStream_t stream, int blkpsm, int* grid_out)
 template <typename V>
 int launch_cg_fmt(int fmt, CgArgs<V>& a, hipStream_t stream, int blkpsm, int* grid_out)
 {
-    return fmt == FMT_CSR ? launch_cg<V, FMT_CSR>(a, stream, blkpsm, grid_out)
-                          : launch_cg<V, FMT_DIA>(a, stream, blkpsm, grid_out);
+    if (fmt == FMT_CSR) return launch_cg<V, FMT_CSR>(a, stream, blkpsm, grid_out);
+    if (a.ndiag == 7) return launch_cg<V, FMT_DIA7>(a, stream, blkpsm, grid_out);
+    return launch_cg<V, FMT_DIA>(a, stream, blkpsm, grid_out);
 }
 
 }  // namespace
@@ -415,7 +425,6 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
         }
         a.part_a = S.parts;
         a.part_b = S.parts + S.nseg;
-        a.barrier = S.barrier;
         a.rr_out = S.small;
         a.bb_out = S.small + 1;
         a.iters_out = (int*)(S.small + 2);
@@ -450,6 +459,13 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
     PERKS_HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0.f;
     PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    GridSync* gs = nullptr;
+    rc = grid_sync_get(kSyncCg, &gs);
+    if (rc != PERKS_OK) return rc;
+    if (!refine) {
+        rc = grid_sync_check(gs, stream);
+        if (rc != PERKS_OK) return rc;
+    }
 
     if (!refine) {
         int it = 0;
@@ -489,6 +505,8 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
             double small[3];
             PERKS_HIP_TRY(hipMemcpyAsync(small, S.small, sizeof(small), hipMemcpyDeviceToHost, stream));
             PERKS_HIP_TRY(hipStreamSynchronize(stream));
+            rc = grid_sync_check(gs, stream);
+            if (rc != PERKS_OK) return rc;
             rr = small[0];
             bb = small[1];
             hist_host.push_back(rr);
@@ -529,7 +547,6 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
             f.max_iter = o->max_inner > 0 ? o->max_inner : o->max_iter;
             f.part_a = S.parts;
             f.part_b = S.parts + S.nseg;
-            f.barrier = S.barrier;
             f.rr_out = S.small;
             f.bb_out = S.small + 1;
             f.iters_out = (int*)(S.small + 2);
@@ -544,6 +561,8 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
             float m2 = 0.f;
             PERKS_HIP_TRY(hipEventElapsedTime(&m2, ev0, ev1));
             elapsed += m2;
+            rc = grid_sync_check(gs, stream);
+            if (rc != PERKS_OK) return rc;
             PERKS_HIP_TRY(hipMemcpy(small, S.small, sizeof(small), hipMemcpyDeviceToHost));
             int it = 0;
             std::memcpy(&it, &small[2], sizeof(int));
@@ -574,7 +593,7 @@ int release_cg_scratch()
     std::lock_guard<std::mutex> lk(g_cg_mu);
     if (dev < 0 || dev >= 64) return PERKS_OK;
     CgScratch& s = g_cg[dev];
-    for (void* p : {s.p0, s.p1, s.ap, s.r, (void*)s.rf, (void*)s.xf, (void*)s.valf, (void*)s.parts, (void*)s.barrier,
+    for (void* p : {s.p0, s.p1, s.ap, s.r, (void*)s.rf, (void*)s.xf, (void*)s.valf, (void*)s.parts,
                     (void*)s.small, (void*)s.hist})
         if (p) PERKS_HIP_TRY(hipFree(p));
     s = CgScratch{};

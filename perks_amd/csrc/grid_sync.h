@@ -1,0 +1,138 @@
+// grid_sync.h -- the grid-wide barrier of the persistent (cooperative) kernels.
+//
+// State lives in one device block per subsystem and device, zeroed once
+// when it is allocated.  The barrier resets itself: the last workgroup to
+// arrive returns the arrival count to zero and then advances the generation,
+// so a launch needs no clearing kernel or memset beforehand (and a launch
+// captured in a hipGraph carries none).
+//
+// Every wait is bounded.  A workgroup that waits longer than `timeout` ticks
+// of the device wall clock sets the abort word and leaves; every other
+// workgroup sees the word in its own wait (or at its next barrier) and leaves
+// too, so the grid always drains.  The host checks the word after the launch
+// (grid_sync_check) and reports PERKS_ERR_BARRIER_TIMEOUT.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace perks {
+
+// Layout: the root counter and the abort word on lines of their own, then
+// kSyncGroups sub-groups, each with an arrival counter and a release word,
+// 4352 bytes apart (17 x 256 B: the groups fall on different HBM channels).
+// Workgroup w belongs to group w % kSyncGroups; the last arrival of a group
+// arrives at the root; the last root arrival releases every group.  At most
+// about G/32 atomics meet on one address, and each group polls its own word.
+// The last arrival of a group resets the group counter, and the releasing
+// arrival resets the root, each with an atomic exchange that completes before
+// the next step is issued, so every barrier leaves all counters at zero and
+// the next launch may use any grid size.
+constexpr int kSyncGroups = 32;
+constexpr int kSyncStride = 4352 / 8;  // in 64-bit words
+
+struct GridSync {
+    unsigned long long w[kSyncStride * (kSyncGroups + 1)];
+};
+static_assert(sizeof(GridSync) == 4352 * 33, "GridSync layout");
+constexpr int kSyncAbortWord = 256;  // index of the abort word (2 KiB into the root block)
+
+__device__ __forceinline__ unsigned int* gs_root(GridSync* gs) { return (unsigned int*)gs->w; }
+__device__ __forceinline__ unsigned int* gs_abort(GridSync* gs) { return (unsigned int*)(gs->w + kSyncAbortWord); }
+__device__ __forceinline__ unsigned int* gs_count(GridSync* gs, int g)
+{
+    return (unsigned int*)(gs->w + kSyncStride * (g + 1));
+}
+
+// Waits until this thread's outstanding memory operations (here: an atomic
+// exchange) have completed, and keeps the compiler from moving memory
+// operations across the wait.
+__device__ __forceinline__ void gs_complete()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ unsigned int* gs_gen(GridSync* gs, int g)
+{
+    return (unsigned int*)(gs->w + kSyncStride * (g + 1) + 16);
+}
+
+// All threads of the workgroup call this.  Every wave waits for its own loads
+// and stores to complete, the workgroup meets, and thread 0 then writes back
+// the L2 at agent scope (so the other XCDs see what the workgroup stored),
+// arrives, waits for the release, and invalidates L1/L2 at agent scope before
+// the workgroup meets again and reads.  Returns false when the grid is
+// aborting: the caller must leave the kernel.
+__device__ __forceinline__ bool grid_sync(GridSync* gs, unsigned int nblocks, unsigned long long timeout)
+{
+    __shared__ int s_ok;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned int ng = nblocks < (unsigned)kSyncGroups ? nblocks : (unsigned)kSyncGroups;
+        const int g = (int)(blockIdx.x % ng);
+        const unsigned int members = (nblocks - (unsigned)g + ng - 1) / ng;
+        unsigned int* gen = gs_gen(gs, g);
+        // The release word is read before arriving (it cannot change without this
+        // arrival); the release fence below also waits for this load.
+        const unsigned int g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned int before =
+            __hip_atomic_fetch_add(gs_count(gs, g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool released = false;
+        if (before == members - 1) {
+            __hip_atomic_exchange(gs_count(gs, g), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            gs_complete();
+            const unsigned int rb = __hip_atomic_fetch_add(gs_root(gs), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (rb == ng - 1) {
+                __hip_atomic_exchange(gs_root(gs), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                gs_complete();
+                for (unsigned int j = 0; j < ng; ++j)
+                    __hip_atomic_fetch_add(gs_gen(gs, (int)j), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                released = true;
+            }
+        }
+        if (!released) {
+            const unsigned long long t0 = wall_clock64();
+            unsigned int spins = 0;
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                // The abort word and the clock are looked at every 64th poll only.
+                if ((++spins & 63u) != 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                if (__hip_atomic_load(gs_abort(gs), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > timeout) {
+                    __hip_atomic_store(gs_abort(gs), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        if (ok && __hip_atomic_load(gs_abort(gs), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// ---- host side ----------------------------------------------------------------
+enum GridSyncUser { kSyncJ3d = 0, kSyncJ2d = 1, kSyncCg = 2, kSyncUsers = 3 };
+
+// The per-device block of `user` (allocated and zeroed on first use, kept
+// until grid_sync_release).
+int grid_sync_get(int user, GridSync** out);
+// Wait limit in wall-clock ticks for one barrier (10 s).
+unsigned long long grid_sync_timeout();
+// After a launch has completed: PERKS_OK, or PERKS_ERR_BARRIER_TIMEOUT (the
+// block is then cleared for the next launch).  Synchronises `stream`.
+int grid_sync_check(GridSync* gs, hipStream_t stream);
+int grid_sync_release();
+
+}  // namespace perks

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "grid_sync.h"
 #include "j3d_common.h"
 
 namespace perks {
@@ -33,7 +34,8 @@ struct J2dArgs {
     int npass;              // passes this launch runs
     int step_offset;        // index of this launch's first pass (per-step launches)
     int levels;             // time steps per full pass
-    unsigned int* barrier;  // grid barrier counter, zeroed before the launch
+    GridSync* sync;               // grid barrier state (grid_sync.h)
+    unsigned long long timeout;   // barrier wait limit, wall-clock ticks
 };
 
 int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);

@@ -54,18 +54,16 @@ void tile_dims(int dtype, int* tx, int* ty)
 }
 
 std::mutex g_mu2;
-unsigned int* g_bar2[64] = {};
 J2dGrid* g_grids[64] = {};
 int* g_active[64] = {};
 size_t g_grids_cap[64] = {}, g_active_cap[64] = {};
 
-int scratch2(int ngrids, int npass, unsigned int** bar, J2dGrid** grids, int** active)
+int scratch2(int ngrids, int npass, J2dGrid** grids, int** active)
 {
     int dev = 0;
     PERKS_HIP_TRY(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return PERKS_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(g_mu2);
-    if (!g_bar2[dev]) PERKS_HIP_TRY(hipMalloc(&g_bar2[dev], 256));
     if (g_grids_cap[dev] < (size_t)ngrids) {
         if (g_grids[dev]) PERKS_HIP_TRY(hipFree(g_grids[dev]));
         PERKS_HIP_TRY(hipMalloc(&g_grids[dev], sizeof(J2dGrid) * (size_t)ngrids));
@@ -76,15 +74,9 @@ int scratch2(int ngrids, int npass, unsigned int** bar, J2dGrid** grids, int** a
         PERKS_HIP_TRY(hipMalloc(&g_active[dev], sizeof(int) * (size_t)npass));
         g_active_cap[dev] = (size_t)npass;
     }
-    *bar = g_bar2[dev];
     *grids = g_grids[dev];
     *active = g_active[dev];
     return PERKS_OK;
-}
-
-__global__ void zero_u32_2(unsigned int* p, int n)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 }  // namespace
@@ -167,10 +159,12 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     }
     // item_end is cumulative over the sorted list, so a prefix of grids covers
     // a prefix of items.  Tiles per grid do not depend on the pass.
-    unsigned int* bar = nullptr;
     J2dGrid* d_grids = nullptr;
     int* d_active = nullptr;
-    rc = scratch2((int)grids.size(), npass, &bar, &d_grids, &d_active);
+    rc = scratch2((int)grids.size(), npass, &d_grids, &d_active);
+    if (rc != PERKS_OK) return rc;
+    GridSync* gs = nullptr;
+    rc = grid_sync_get(kSyncJ2d, &gs);
     if (rc != PERKS_OK) return rc;
     PERKS_HIP_TRY(hipMemcpyAsync(d_grids, grids.data(), sizeof(J2dGrid) * grids.size(), hipMemcpyHostToDevice,
                                  stream));
@@ -182,13 +176,12 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     a.grids = d_grids;
     a.active = d_active;
     a.levels = L;
-    a.barrier = bar;
+    a.sync = gs;
+    a.timeout = grid_sync_timeout();
 
     hipEvent_t ev0, ev1;
     PERKS_HIP_TRY(hipEventCreate(&ev0));
     PERKS_HIP_TRY(hipEventCreate(&ev1));
-    hipLaunchKernelGGL(zero_u32_2, dim3(1), dim3(64), 0, stream, bar, 64);
-    PERKS_HIP_TRY(hipGetLastError());
     int grid_launched = 0, occ_used = 0;
     PERKS_HIP_TRY(hipEventRecord(ev0, stream));
     if (o->method == PERKS_METHOD_PLAIN) {
@@ -223,6 +216,10 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
+    if (o->method != PERKS_METHOD_PLAIN) {
+        rc = grid_sync_check(gs, stream);
+        if (rc != PERKS_OK) return rc;
+    }
     if (rep) {
         std::memset(rep, 0, sizeof(*rep));
         rep->elapsed_ms = ms;
@@ -245,10 +242,8 @@ int release_j2d_scratch()
     PERKS_HIP_TRY(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_mu2);
     if (dev < 0 || dev >= 64) return PERKS_OK;
-    if (g_bar2[dev]) PERKS_HIP_TRY(hipFree(g_bar2[dev]));
     if (g_grids[dev]) PERKS_HIP_TRY(hipFree(g_grids[dev]));
     if (g_active[dev]) PERKS_HIP_TRY(hipFree(g_active[dev]));
-    g_bar2[dev] = nullptr;
     g_grids[dev] = nullptr;
     g_active[dev] = nullptr;
     g_grids_cap[dev] = g_active_cap[dev] = 0;

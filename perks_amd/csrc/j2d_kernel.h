@@ -119,20 +119,6 @@ __device__ __forceinline__ C eval_shape(const C (&k)[kMaxGroups], double kd_div,
 }
 
 // ---- the kernel ---------------------------------------------------------------
-__device__ __forceinline__ void grid_barrier2(unsigned int* counter, unsigned int nblocks, unsigned int epoch)
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned int target = epoch * nblocks;
-        while ((int)(__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0)
-            __builtin_amdgcn_s_sleep(2);
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
-
 template <int SHAPE, typename S, int BC, int TX, int TY, bool HAS_DIV>
 __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
 {
@@ -226,7 +212,7 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
                 if (y < ny && x < nx) Fmt<S>::st(dst, (size_t)y * nx + x, tile[fin][ly][lx]);
             }
         }
-        if (pass + 1 < a.npass) grid_barrier2(a.barrier, G, (unsigned int)(pass + 1));
+        if (pass + 1 < a.npass && !grid_sync(a.sync, G, a.timeout)) return;
     }
 }
 

@@ -49,11 +49,6 @@ using j3dp::kVariants;
 using j3dp::PersistArgs;
 using j3dp::Variant;
 
-__global__ void zero_u32(unsigned int* p, int n)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
-}
-
 template <typename T>
 void* pick_kernel(int bc, bool has_div, int var)
 {
@@ -64,21 +59,6 @@ int variant_of(const perks_stencil_opts* o)
 {
     const int v = o->tune[4];
     return (v >= 0 && v < kNumVariants) ? v : 0;
-}
-
-// Device scratch for the grid barrier, one per device, kept until perks_release().
-std::mutex g_scratch_mu;
-unsigned int* g_barrier[64] = {};
-
-int barrier_scratch(unsigned int** out)
-{
-    int dev = 0;
-    PERKS_HIP_TRY(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) return PERKS_ERR_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
-    if (!g_barrier[dev]) PERKS_HIP_TRY(hipMalloc(&g_barrier[dev], 256));
-    *out = g_barrier[dev];
-    return PERKS_OK;
 }
 
 }  // namespace
@@ -163,8 +143,8 @@ int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoe
     rc = plan_persistent(dtype, nz, ny, nx, o, &caps, occ, &pl);
     if (rc != PERKS_OK) return rc;
 
-    unsigned int* bar = nullptr;
-    rc = barrier_scratch(&bar);
+    GridSync* gs = nullptr;
+    rc = grid_sync_get(kSyncJ3d, &gs);
     if (rc != PERKS_OK) return rc;
 
     PersistArgs a;
@@ -175,15 +155,14 @@ int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoe
     a.steps = steps;
     a.levels = pl.reserved[0];
     for (int l = 1; l <= kMaxLevels; ++l) a.nzs[l] = pl.reserved[l];
-    a.barrier = bar;
+    a.sync = gs;
+    a.timeout = grid_sync_timeout();
     KCoefs kc = cf;
     void* args[] = {&d_a, &d_b, &a, &kc};
 
     hipEvent_t ev0, ev1;
     PERKS_HIP_TRY(hipEventCreate(&ev0));
     PERKS_HIP_TRY(hipEventCreate(&ev1));
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, stream, bar, 64);
-    PERKS_HIP_TRY(hipGetLastError());
     PERKS_HIP_TRY(hipEventRecord(ev0, stream));
     if (steps > 0)
         PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(pl.grid[0]), dim3(pl.block), args, 0, stream));
@@ -193,6 +172,8 @@ int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoe
     PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
+    rc = grid_sync_check(gs, stream);
+    if (rc != PERKS_OK) return rc;
     if (rep) {
         const int npass = steps > 0 ? (steps + a.levels - 1) / a.levels : 0;
         rep->elapsed_ms = ms;
@@ -217,14 +198,7 @@ template int run_persistent<float>(float*, float*, int, int, int, int, const KCo
 
 int release_persistent_scratch()
 {
-    int dev = 0;
-    PERKS_HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
-    if (dev >= 0 && dev < 64 && g_barrier[dev]) {
-        PERKS_HIP_TRY(hipFree(g_barrier[dev]));
-        g_barrier[dev] = nullptr;
-    }
-    return PERKS_OK;
+    return grid_sync_release();
 }
 
 }  // namespace perks

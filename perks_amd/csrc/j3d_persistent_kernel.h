@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "grid_sync.h"
 #include "j3d_common.h"
 
 namespace perks {
@@ -19,26 +20,9 @@ struct PersistArgs {
     int steps;
     int levels;                 // time steps per pass (L of the full passes)
     int nzs[kMaxLevels + 1];    // z-slabs per pass with l levels
-    unsigned int* barrier;      // zeroed before the launch
+    GridSync* sync;             // grid barrier state (grid_sync.h)
+    unsigned long long timeout; // barrier wait limit, wall-clock ticks
 };
-
-// Grid-wide barrier for a cooperative launch.  Each wave releases its own
-// stores at agent scope (s_waitcnt + L2 write-back), the workgroup meets, one
-// lane counts the arrival, spins until every workgroup of this epoch has
-// arrived, and every wave acquires (L2 invalidate) before reading again.
-__device__ __forceinline__ void grid_barrier(unsigned int* counter, unsigned int nblocks, unsigned int epoch)
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned int target = epoch * nblocks;
-        while ((int)(__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0)
-            __builtin_amdgcn_s_sleep(2);
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
 
 template <typename T, int BC>
 __device__ __forceinline__ T bc_value(T c)
@@ -293,7 +277,7 @@ j3d_star1_persistent_kernel(T* buf_a, T* buf_b, PersistArgs a, KCoefs cf)
         T* t = src;
         src = dst;
         dst = t;
-        if (pass + 1 < npass) grid_barrier(a.barrier, gridDim.x, (unsigned int)(pass + 1));
+        if (pass + 1 < npass && !grid_sync(a.sync, gridDim.x, a.timeout)) return;
     }
 }
 
