@@ -15,6 +15,9 @@ time steps * cells / seconds, the reference's own definition
 of the field per cell-update (j3d.cu:756) against 8 TB/s of HBM.
 
 Other workloads (--workload; each prints its own JSON line):
+  j3d-storage  the same 3D solve with FP64 / FP32 / FP16 / BF16 storage (FP32
+               arithmetic for the 16-bit forms), persistent and per-step
+               kernels, with max and RMS error against the FP64 result.
   j2d-storage  2D box r1 (2d9pt) and r2 (2d25pt), 8192^2, 500 steps, storage
                FP32 / FP16 / BF16 with FP32 arithmetic, against FP64 on the
                same inputs (max and RMS error at 10 / 100 / 500 steps).
@@ -54,7 +57,7 @@ def parse():
     ap.add_argument("--no-per-step", action="store_true", help="skip the per-step comparison")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
-    ap.add_argument("--workload", choices=["j3d", "j2d-storage", "batch2d", "cg"], default="j3d")
+    ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j2d-storage", "batch2d", "cg"], default="j3d")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
@@ -189,6 +192,74 @@ def run_j3d(args):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------- 3D storage precision
+def run_j3d_storage(args):
+    import torch
+    import perks_amd as P
+
+    n, T = args.size, args.tsteps
+    cells = n ** 3
+    storage = {"f64": torch.float64, "f32": torch.float32, "f16": torch.int16, "bf16": torch.int16}
+    view = {"f16": torch.float16, "bf16": torch.bfloat16}
+    rows, ref = {}, None
+    forms = [("f64", "nearest"), ("f32", "nearest"), ("bf16", "nearest"), ("f16", "nearest"), ("bf16", "stochastic"),
+             ("f16", "stochastic")]
+    for dt, rounding in forms:
+        name = dt if rounding == "nearest" else f"{dt}_sr"
+        a = torch.empty(cells, dtype=storage[dt], device="cuda")
+        b = torch.empty_like(a)
+        init = torch.empty_like(a)
+        P.fill_uniform(init, cells, args.seed, 0.0, 1.0, dtype=dt)
+        row = {}
+        for method in ("persistent", "plain"):
+            times = []
+            for _ in range(max(1, args.steps)):
+                a.copy_(init)
+                torch.cuda.synchronize()
+                rep = P.j3d_device(a, b, n, n, n, T, dtype=dt, method=method, boundary="zero", coefs=CONFIG_COEFS,
+                                   rounding=rounding, round_seed=args.seed)
+                times.append(rep.elapsed_ms)
+            ms = sum(times) / len(times)
+            row[method] = {"gcells_per_s": round(T * cells / (ms / 1e3) / 1e9, 2), "ms_per_solve": round(ms, 3)}
+            res = b if rep.result_in_b else a
+            val = res.view(view[dt]).double() if dt in view else res.double()
+            if dt == "f64" and method == "persistent":
+                ref = val.clone()
+            elif method == "persistent":
+                row["error_vs_f64"] = _errors(torch, val, ref)
+            if method == "plain":
+                prev = row.get("_persistent_result")
+                row["plain_equals_persistent"] = bool(torch.equal(prev, res)) if prev is not None else None
+            else:
+                row["_persistent_result"] = res.clone()
+        row.pop("_persistent_result", None)
+        rows[name] = row
+        del a, b, init
+        torch.cuda.empty_cache()
+    for name in rows:
+        for m in ("persistent", "plain"):
+            rows[name][m]["speedup_vs_f32"] = round(rows[name][m]["gcells_per_s"] / rows["f32"][m]["gcells_per_s"], 3)
+    v = rows["bf16_sr"]["plain"]
+    alg = T * cells * 2 * 2
+    achieved = alg / (v["ms_per_solve"] / 1e3) / 1e9
+    out = {
+        "metric": "GCells/s, 3d7pt 512^3 1000 steps: FP64 / FP32 / FP16 / BF16 storage, persistent and per-step, "
+                  "error vs FP64",
+        "value": v["gcells_per_s"], "unit": "GCells/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": v["ms_per_solve"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 storage / f32 arithmetic", "data": "synthetic",
+        "config": {"workload": f"3d7pt star Jacobi, {n}^3 interior + zero halo, c=0.4 / 6x0.1, input uniform[0,1) "
+                               f"(seed {args.seed}), {T} time steps; value = BF16 storage with stochastic "
+                               "rounding, per-step kernel",
+                   "size": n, "time_steps": T, "results": rows},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "j3d_star1_plain16_kernel<bf16, stochastic> (x 1000 launches)",
+                     "algorithmic_bytes_per_launch": alg},
+    }
+    print(json.dumps(out), flush=True)
+
+
 # ---------------------------------------------------------------------------- 2D storage precision
 def _errors(torch, got, ref):
     """max and RMS of |got - ref|, absolute and relative to max|ref| / RMS(ref)."""
@@ -217,12 +288,14 @@ def run_j2d_storage(args):
         coefs = list(c / c.sum())
         ref_at = {}
         per = {}
-        for dt in ("f64", "f32", "f16", "bf16"):
+        for dt, rounding in (("f64", "nearest"), ("f32", "nearest"), ("f16", "nearest"), ("bf16", "nearest"),
+                             ("f16", "stochastic"), ("bf16", "stochastic")):
+            name = dt if rounding == "nearest" else f"{dt}_sr"
             a = torch.empty(cells, dtype=storage[dt], device="cuda")
             b = torch.empty_like(a)
             init = torch.empty_like(a)
             P.fill_uniform(init, cells, args.seed, -1.0, 1.0, dtype=dt)
-            opts = P.make_opts(shape, "clamp", "persistent", coefs)
+            opts = P.make_opts(shape, "clamp", "persistent", coefs, rounding=rounding, round_seed=args.seed)
             row = {}
             for steps in checkpoints:
                 a.copy_(init)
@@ -245,17 +318,17 @@ def run_j2d_storage(args):
                     row["gcells_per_s"] = round(T * cells / (ms / 1e3) / 1e9, 2)
                     row["ms_per_solve"] = round(ms, 3)
                     row["steps_per_pass"] = rep.reserved[0]
-                    plain = P.make_opts(shape, "clamp", "plain", coefs)
+                    plain = P.make_opts(shape, "clamp", "plain", coefs, rounding=rounding, round_seed=args.seed)
                     a.copy_(init)
                     torch.cuda.synchronize()
                     pr = P.jacobi2d_device(a, b, n, n, T, dtype=dt, opts=plain)
                     row["per_step_kernel_gcells_per_s"] = round(T * cells / (pr.elapsed_ms / 1e3) / 1e9, 2)
-            per[dt] = row
+            per[name] = row
             del a, b, init
         del ref_at
         torch.cuda.empty_cache()
-        for dt in ("f32", "f16", "bf16"):
-            per[dt]["speedup_vs_f32"] = round(per[dt]["gcells_per_s"] / per["f32"]["gcells_per_s"], 3)
+        for name in per:
+            per[name]["speedup_vs_f32"] = round(per[name]["gcells_per_s"] / per["f32"]["gcells_per_s"], 3)
         results[shape] = per
     bf = results["2d9pt"]["bf16"]
     ms = kernel_ms[("2d9pt", "bf16")]
@@ -444,7 +517,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
-    {"j3d": run_j3d, "j2d-storage": run_j2d_storage, "batch2d": run_batch2d, "cg": run_cg}[args.workload](args)
+    {"j3d": run_j3d, "j3d-storage": run_j3d_storage, "j2d-storage": run_j2d_storage, "batch2d": run_batch2d,
+     "cg": run_cg}[args.workload](args)
 
 
 if __name__ == "__main__":

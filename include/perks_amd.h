@@ -48,6 +48,7 @@ extern "C" {
 #define PERKS_ERR_BARRIER_TIMEOUT (-15) /* a grid barrier wait exceeded 10 s; launch abandoned */
 
 /* ---- enums ---------------------------------------------------------------- */
+enum perks_rounding { PERKS_ROUND_NEAREST = 0, PERKS_ROUND_STOCHASTIC = 1 };
 enum perks_dtype { PERKS_F64 = 0, PERKS_F32 = 1, PERKS_F16 = 2, PERKS_BF16 = 3 };
 
 /* Shapes are named by (dimension, kind, radius).  Reference directory names:
@@ -100,6 +101,16 @@ typedef struct perks_stencil_opts {
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
      *   tune[4] persistent: tile variant (0..3)      tune[5..7] unused */
     int32_t tune[8];
+    /* FP16/BF16 storage only (no reference counterpart): how each step's FP32
+     * result is rounded to the storage format.  PERKS_ROUND_NEAREST (default):
+     * nearest even.  PERKS_ROUND_STOCHASTIC: up or down with probability
+     * proportional to the distance, the random bits being a counter hash of
+     * (cell index, time step, round_seed), so runs are reproducible and the
+     * per-step and persistent methods agree bit for bit.  Nearest rounding
+     * drops every change smaller than half a unit in the last place, so a
+     * slowly decaying field stalls; stochastic rounding keeps it unbiased. */
+    int32_t rounding;
+    uint32_t round_seed;
 } perks_stencil_opts;
 
 typedef struct perks_run_report {
@@ -178,6 +189,13 @@ int perks_j3d_device_f64(double* d_a, double* d_b, int height, int width_y, int 
                          const perks_stencil_opts* opts, void* stream, perks_run_report* report);
 int perks_j3d_device_f32(float* d_a, float* d_b, int height, int width_y, int width_x, int steps,
                          const perks_stencil_opts* opts, void* stream, perks_run_report* report);
+/* Any storage type (enum perks_dtype): FP64, FP32, or FP16/BF16 storage (raw
+ * 16-bit patterns) with FP32 arithmetic, every step's result rounded to the
+ * storage format (nearest even).  The per-step method needs an even width_x
+ * for 16-bit storage (PERKS_ERR_UNSUPPORTED_SIZE otherwise).  No reference
+ * counterpart: the reduced-precision form is an addition of this library. */
+int perks_j3d_device(int dtype, void* d_a, void* d_b, int height, int width_y, int width_x, int steps,
+                     const perks_stencil_opts* opts, void* stream, perks_run_report* report);
 
 /* ---- 2D Jacobi ------------------------------------------------------------ */
 /* Host buffers h_in[width_y][width_x] (x fastest), as the reference's

@@ -256,6 +256,59 @@ void po_stencil_iterate_bf16s(const uint16_t* in, uint16_t* out, int nz, int ny,
     iterate_u16(in, out, nz, ny, nx, st, steps, 1);
 }
 
+/* ---- stochastic rounding of 16-bit storage (see perks_oracle.h) ---------- */
+uint32_t po_lowbias32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+uint32_t po_sr_bits(uint32_t idx, uint32_t step, uint32_t seed)
+{
+    return po_lowbias32(idx ^ po_lowbias32(seed ^ (step * 0x9E3779B9u)));
+}
+
+uint16_t po_f32_to_bf16_sr(float f, uint32_t r)
+{
+    const uint32_t u = f2u(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + (r & 0xFFFFu)) >> 16);
+}
+
+uint16_t po_f32_to_f16_sr(float f, uint32_t r)
+{
+    const uint32_t u = f2u(f);
+    const uint32_t e = (u >> 23) & 0xFFu;
+    if (e < 113u || e == 255u) return po_f32_to_f16(f);
+    const uint32_t t = (u + (r & 0x1FFFu)) & 0xFFFFE000u;
+    if ((t & 0x7FFFFFFFu) > 0x477FE000u) return (uint16_t)(((u >> 16) & 0x8000u) | 0x7C00u);
+    return po_f32_to_f16(u2f(t)); /* exactly representable: conversion is exact */
+}
+
+void po_stencil_iterate_sr16(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st,
+                             int steps, int fmt, uint32_t seed)
+{
+    const size_t n = (size_t)nz * ny * nx;
+    float* a = (float*)malloc(n * sizeof(float));
+    float* b = (float*)malloc(n * sizeof(float));
+    for (size_t i = 0; i < n; ++i) a[i] = fmt ? po_bf16_to_f32(in[i]) : po_f16_to_f32(in[i]);
+    for (int s = 0; s < steps; ++s) {
+        po_stencil_step_f32(a, b, nz, ny, nx, st);
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t r = po_sr_bits((uint32_t)i, (uint32_t)s, seed);
+            const uint16_t q = fmt ? po_f32_to_bf16_sr(b[i], r) : po_f32_to_f16_sr(b[i], r);
+            a[i] = fmt ? po_bf16_to_f32(q) : po_f16_to_f32(q);
+        }
+    }
+    for (size_t i = 0; i < n; ++i) out[i] = fmt ? po_f32_to_bf16(a[i]) : po_f32_to_f16(a[i]);
+    free(a);
+    free(b);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Sparse matrices                                                           */
 /* ------------------------------------------------------------------------ */

@@ -85,6 +85,22 @@ float po_bf16_to_f32(uint16_t b);
 void po_stencil_iterate_f16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps);
 void po_stencil_iterate_bf16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps);
 
+/* Stochastic rounding of 16-bit storage (this library's addition; no
+ * reference counterpart).  The random bits of cell `idx` (linear index
+ * z*ny*nx + y*nx + x, taken mod 2^32) at time step t (0-based) are
+ * lowbias32(idx ^ lowbias32(seed ^ t*0x9E3779B9)), where lowbias32 is
+ * C. Wellons' public-domain integer hash.  BF16: (bits(v) + (r & 0xFFFF)) >> 16
+ * (NaN -> quiet NaN).  FP16: in the FP16 normal range, the 13 dropped bits get
+ * (r & 0x1FFF) added before truncation (results above 65504 -> inf); below it,
+ * at zero, inf and NaN, round to nearest even. */
+uint32_t po_lowbias32(uint32_t x);
+uint32_t po_sr_bits(uint32_t idx, uint32_t step, uint32_t seed);
+uint16_t po_f32_to_bf16_sr(float f, uint32_t r);
+uint16_t po_f32_to_f16_sr(float f, uint32_t r);
+/* fmt: 0 = FP16, 1 = BF16. */
+void po_stencil_iterate_sr16(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st,
+                             int steps, int fmt, uint32_t seed);
+
 /* RMS and max |a-b| over a box [z0,z1)x[y0,y1)x[x0,x1) (common.hpp:133-163). */
 double po_rms_error_f64(const double* a, const double* b, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1, double* max_err);
 double po_rms_error_f32(const float* a, const float* b, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1, double* max_err);

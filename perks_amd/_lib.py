@@ -52,6 +52,7 @@ class StencilOpts(C.Structure):
         ("bdimx", C.c_int32), ("blkpsm", C.c_int32), ("use_lds_cache", C.c_int32),
         ("reg_cache_planes", C.c_int32), ("double_tile", C.c_int32), ("warmup", C.c_int32),
         ("warmup_steps", C.c_int32), ("tune", C.c_int32 * 8),
+        ("rounding", C.c_int32), ("round_seed", C.c_uint32),
     ]
 
 
@@ -129,6 +130,7 @@ SIGNATURES = {
     "perks_j3d_iterative_f32": (_I, [_P, _I, _I, _I, _P, _I, C.POINTER(StencilOpts), C.POINTER(RunReport)]),
     "perks_j3d_device_f64": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
     "perks_j3d_device_f32": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
+    "perks_j3d_device": (_I, [_I, _P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
     "perks_fill_uniform_f64": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
     "perks_fill_uniform_f32": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
     "perks_fill_uniform_f16": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),

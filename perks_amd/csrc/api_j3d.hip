@@ -14,6 +14,7 @@
 
 #include "api_common.h"
 #include "j3d_kernels.h"
+#include "storage16.h"
 #include "j3d_persistent.h"
 #include "planner.h"
 #include "j2d.h"
@@ -94,9 +95,12 @@ int validate(const T* a, const T* b, int nz, int ny, int nx, int steps, const pe
 {
     if (!a || !b || !o) return PERKS_ERR_INVALID_ARG;
     if (nz <= 0 || ny <= 0 || nx <= 0 || steps < 0) return PERKS_ERR_INVALID_ARG;
-    if ((long long)ny * nx > 0x7fffffffLL) return PERKS_ERR_UNSUPPORTED_SIZE;
+    // One z-plane is addressed through a buffer descriptor with 32-bit byte
+    // offsets, and offset 2^31 marks "no access": a plane must stay below 2 GiB.
+    if ((long long)ny * nx * (long long)sizeof(T) >= (long long)kOob) return PERKS_ERR_UNSUPPORTED_SIZE;
     if (o->boundary != PERKS_BC_CLAMP && o->boundary != PERKS_BC_ZERO) return PERKS_ERR_INVALID_ARG;
     if (o->method != PERKS_METHOD_PLAIN && o->method != PERKS_METHOD_PERSISTENT) return PERKS_ERR_INVALID_ARG;
+    if (o->rounding != PERKS_ROUND_NEAREST && o->rounding != PERKS_ROUND_STOCHASTIC) return PERKS_ERR_INVALID_ARG;
     const ShapeDesc* s = shape_by_id(o->shape);
     if (!s || s->dims != 3) return PERKS_ERR_INVALID_ARG;
     if (o->shape != PERKS_SHAPE_3D_STAR_R1) return PERKS_ERR_UNSUPPORTED;
@@ -116,6 +120,65 @@ int j3d_device(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const perks_st
     hipStream_t s = (hipStream_t)stream;
     if (o->method == PERKS_METHOD_PLAIN) return run_plain<T>(d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
     return run_persistent<T>(d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
+}
+
+// FP16/BF16 storage (16-bit patterns), FP32 arithmetic.
+int run_plain16(int fmt, uint16_t* d_a, uint16_t* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
+                const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep)
+{
+    if ((nx & 1) != 0) return PERKS_ERR_UNSUPPORTED_SIZE;
+    perks_device_caps caps;
+    int rc = device_caps_cached(&caps);
+    if (rc != PERKS_OK) return rc;
+    perks_plan_info pl;
+    rc = plan_plain(nz, ny, nx / 2, caps.cu_count, &pl);  // a lane holds two cells
+    if (rc != PERKS_OK) return rc;
+    const int zchunk = o->tune[0] > 0 ? o->tune[0] : pl.reserved[0];
+    hipEvent_t ev0, ev1;
+    PERKS_HIP_TRY(hipEventCreate(&ev0));
+    PERKS_HIP_TRY(hipEventCreate(&ev1));
+    PlainGeom geom{0, 0, 0};
+    PERKS_HIP_TRY(hipEventRecord(ev0, stream));
+    uint16_t* src = d_a;
+    uint16_t* dst = d_b;
+    for (int s = 0; s < steps && rc == PERKS_OK; ++s) {
+        const bool sr = o->rounding == PERKS_ROUND_STOCHASTIC;
+        rc = launch_j3d_star1_plain16(fmt, src, dst, nz, ny, nx, cf, o->boundary, zchunk, sr,
+                                      sr_key((uint32_t)s, o->round_seed), stream, &geom);
+        uint16_t* t = src;
+        src = dst;
+        dst = t;
+    }
+    PERKS_HIP_TRY(hipEventRecord(ev1, stream));
+    PERKS_HIP_TRY(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (rc != PERKS_OK) return rc;
+    if (rep) {
+        rep->elapsed_ms = ms;
+        rep->gcells_per_s = ms > 0 ? (double)steps * nz * ny * (double)nx / ms / 1e6 : 0.0;
+        rep->grid[0] = geom.grid;
+        rep->grid[1] = rep->grid[2] = 1;
+        rep->block = geom.block;
+        rep->result_in_b = steps % 2 == 1;
+    }
+    return PERKS_OK;
+}
+
+int j3d_device16(int fmt, uint16_t* d_a, uint16_t* d_b, int nz, int ny, int nx, int steps,
+                 const perks_stencil_opts* o, void* stream, perks_run_report* rep)
+{
+    int rc = validate<uint16_t>(d_a, d_b, nz, ny, nx, steps, o);
+    if (rc != PERKS_OK) return rc;
+    KCoefs cf;
+    rc = resolve_coefs(o, &cf, nullptr);
+    if (rc != PERKS_OK) return rc;
+    if (rep) std::memset(rep, 0, sizeof(*rep));
+    hipStream_t s = (hipStream_t)stream;
+    if (o->method == PERKS_METHOD_PLAIN) return run_plain16(fmt, d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
+    return run_persistent16(fmt, d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
 }
 
 template <typename T>
@@ -266,6 +329,24 @@ int perks_j3d_device_f32(float* d_a, float* d_b, int height, int width_y, int wi
                          const perks_stencil_opts* opts, void* stream, perks_run_report* report)
 {
     return perks::j3d_device<float>(d_a, d_b, height, width_y, width_x, steps, opts, stream, report);
+}
+
+int perks_j3d_device(int dtype, void* d_a, void* d_b, int height, int width_y, int width_x, int steps,
+                     const perks_stencil_opts* opts, void* stream, perks_run_report* report)
+{
+    switch (dtype) {
+    case PERKS_F64:
+        return perks::j3d_device<double>((double*)d_a, (double*)d_b, height, width_y, width_x, steps, opts, stream,
+                                         report);
+    case PERKS_F32:
+        return perks::j3d_device<float>((float*)d_a, (float*)d_b, height, width_y, width_x, steps, opts, stream,
+                                        report);
+    case PERKS_F16:
+    case PERKS_BF16:
+        return perks::j3d_device16(dtype, (uint16_t*)d_a, (uint16_t*)d_b, height, width_y, width_x, steps, opts,
+                                   stream, report);
+    default: return PERKS_ERR_INVALID_ARG;
+    }
 }
 
 }  // extern "C"

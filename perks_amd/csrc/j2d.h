@@ -36,6 +36,8 @@ struct J2dArgs {
     int levels;             // time steps per full pass
     GridSync* sync;               // grid barrier state (grid_sync.h)
     unsigned long long timeout;   // barrier wait limit, wall-clock ticks
+    int sr;                       // stochastic rounding of 16-bit storage
+    uint32_t seed;
 };
 
 int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);

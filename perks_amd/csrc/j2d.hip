@@ -105,6 +105,7 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     const ShapeDesc* sd = shape_by_id(o->shape);
     if (!sd || sd->dims != 2) return PERKS_ERR_INVALID_ARG;
     if (o->boundary != PERKS_BC_CLAMP && o->boundary != PERKS_BC_ZERO) return PERKS_ERR_INVALID_ARG;
+    if (o->rounding != PERKS_ROUND_NEAREST && o->rounding != PERKS_ROUND_STOCHASTIC) return PERKS_ERR_INVALID_ARG;
     KCoefs cf;
     int rc = resolve_coefs(o, &cf, nullptr);
     if (rc != PERKS_OK) return rc;
@@ -178,6 +179,8 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     a.levels = L;
     a.sync = gs;
     a.timeout = grid_sync_timeout();
+    a.sr = (dtype == PERKS_F16 || dtype == PERKS_BF16) && o->rounding == PERKS_ROUND_STOCHASTIC;
+    a.seed = o->round_seed;
 
     hipEvent_t ev0, ev1;
     PERKS_HIP_TRY(hipEventCreate(&ev0));

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "j2d.h"
+#include "storage16.h"
 
 namespace perks {
 namespace j2dk {
@@ -33,6 +34,7 @@ struct Fmt<double> {
     __device__ static C ld(const double* p, size_t i) { return p[i]; }
     __device__ static void st(double* p, size_t i, C v) { p[i] = v; }
     __device__ static C q(C v) { return v; }
+    __device__ static C qsr(C v, uint32_t, uint32_t) { return v; }
 };
 template <>
 struct Fmt<float> {
@@ -40,6 +42,7 @@ struct Fmt<float> {
     __device__ static C ld(const float* p, size_t i) { return p[i]; }
     __device__ static void st(float* p, size_t i, C v) { p[i] = v; }
     __device__ static C q(C v) { return v; }
+    __device__ static C qsr(C v, uint32_t, uint32_t) { return v; }
 };
 template <>
 struct Fmt<f16s> {
@@ -48,6 +51,11 @@ struct Fmt<f16s> {
     __device__ static void st(f16s* p, size_t i, C v) { p[i].v = __half_as_ushort(__float2half_rn(v)); }
     // The value after a round trip through storage (every step is stored).
     __device__ static C q(C v) { return __half2float(__float2half_rn(v)); }
+    // Stochastic rounding of cell idx at the step with hash key `key`.
+    __device__ static C qsr(C v, uint32_t idx, uint32_t key)
+    {
+        return Store16<PERKS_F16>::widen(narrow_as<PERKS_F16, true>(v, idx, key));
+    }
 };
 template <>
 struct Fmt<bf16s> {
@@ -62,6 +70,10 @@ struct Fmt<bf16s> {
     }
     __device__ static void st(bf16s* p, size_t i, C v) { p[i].v = bits(v); }
     __device__ static C q(C v) { return __uint_as_float((uint32_t)bits(v) << 16); }
+    __device__ static C qsr(C v, uint32_t idx, uint32_t key)
+    {
+        return Store16<PERKS_BF16>::widen(narrow_as<PERKS_BF16, true>(v, idx, key));
+    }
 };
 
 // ---- one cell update from the shape table ------------------------------------
@@ -181,6 +193,8 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
             const bool edge = dxlo > 0 || dylo > 0 || dxhi < TX - 1 || dyhi < TY - 1;
             for (int l = 0; l < L; ++l) {
                 const int cur = l & 1;
+                // stochastic rounding key of this level's time step (uniform)
+                const uint32_t key = a.sr ? sr_key((uint32_t)(gp * a.levels + l), a.seed) : 0u;
                 for (int i = tid; i < (TX - 2 * R) * (TY - 2 * R); i += 256) {
                     const int ly = R + i / (TX - 2 * R), lx = R + i % (TX - 2 * R);
                     C v;
@@ -201,7 +215,12 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
                         });
                     }
                     // Each step is a stored field: round to the storage format.
-                    tile[cur ^ 1][ly][lx] = Fmt<S>::q(v);
+                    if (a.sr) {
+                        const uint32_t idx = (uint32_t)(oy + ly) * (uint32_t)nx + (uint32_t)(ox + lx);
+                        tile[cur ^ 1][ly][lx] = Fmt<S>::qsr(v, idx, key);
+                    } else {
+                        tile[cur ^ 1][ly][lx] = Fmt<S>::q(v);
+                    }
                 }
                 __syncthreads();
             }

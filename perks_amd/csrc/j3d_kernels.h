@@ -20,4 +20,10 @@ template <typename T>
 int launch_j3d_star1_plain(const T* in, T* out, int nz, int ny, int nx, const KCoefs& cf, int bc, int zchunk,
                            hipStream_t stream, PlainGeom* geom, int variant);
 
+// The same step on FP16 (fmt PERKS_F16) or BF16 (PERKS_BF16) storage with FP32
+// arithmetic; nx must be even.
+// sr: stochastic rounding with the time step's hash key (storage16.h).
+int launch_j3d_star1_plain16(int fmt, const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const KCoefs& cf,
+                             int bc, int zchunk, bool sr, uint32_t key, hipStream_t stream, PlainGeom* geom);
+
 }  // namespace perks

@@ -11,6 +11,10 @@ template <typename T>
 int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
                    const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);
 
+// FP16 (fmt PERKS_F16) or BF16 (PERKS_BF16) storage, FP32 arithmetic.
+int run_persistent16(int fmt, uint16_t* d_a, uint16_t* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
+                     const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);
+
 int release_persistent_scratch();
 
 }  // namespace perks

@@ -49,11 +49,18 @@ using j3dp::kVariants;
 using j3dp::PersistArgs;
 using j3dp::Variant;
 
-template <typename T>
-void* pick_kernel(int bc, bool has_div, int var)
+void* pick_kernel(int fmt, int bc, bool has_div, int var, bool sr)
 {
-    return sizeof(T) == 8 ? j3d_persistent_kernel_f64(bc, has_div, var) : j3d_persistent_kernel_f32(bc, has_div, var);
+    switch (fmt) {
+    case PERKS_F64: return j3d_persistent_kernel_f64(bc, has_div, var, sr);
+    case PERKS_F32: return j3d_persistent_kernel_f32(bc, has_div, var, sr);
+    case PERKS_F16: return j3d_persistent_kernel_f16(bc, has_div, var, sr);
+    case PERKS_BF16: return j3d_persistent_kernel_bf16(bc, has_div, var, sr);
+    default: return nullptr;
+    }
 }
+
+int elem_bytes(int fmt) { return fmt == PERKS_F64 ? 8 : (fmt == PERKS_F32 ? 4 : 2); }
 
 int variant_of(const perks_stencil_opts* o)
 {
@@ -94,7 +101,7 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
                     int occupancy, perks_plan_info* out)
 {
     if (o->shape != PERKS_SHAPE_3D_STAR_R1) return PERKS_ERR_UNSUPPORTED;
-    if ((long long)ny * nx * (dtype == PERKS_F64 ? 8 : 4) >= 0x7fffffffLL) return PERKS_ERR_UNSUPPORTED_SIZE;
+    if ((long long)ny * nx * elem_bytes(dtype) >= (long long)kOob) return PERKS_ERR_UNSUPPORTED_SIZE;
     if (occupancy <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
     int occ = occupancy;
     if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
@@ -123,9 +130,10 @@ int min_width_y_persistent(int, int, int, int, int) { return PERKS_ERR_UNSUPPORT
 
 // ---- launch -----------------------------------------------------------------
 
-template <typename T>
-int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf, const perks_stencil_opts* o,
-                   hipStream_t stream, perks_run_report* rep)
+namespace {
+
+int run_persistent_fmt(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
+                       const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep)
 {
     if (o->shape != PERKS_SHAPE_3D_STAR_R1) return PERKS_ERR_UNSUPPORTED;
     perks_device_caps caps;
@@ -134,12 +142,12 @@ int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoe
     if (!caps.cooperative) return PERKS_ERR_NOT_LAUNCHABLE;
     const bool has_div = cf.divisor != 0.0;
     const int var = variant_of(o);
-    void* fn = pick_kernel<T>(o->boundary, has_div, var);
+    void* fn = pick_kernel(dtype, o->boundary, has_div, var, o->rounding == PERKS_ROUND_STOCHASTIC);
+    if (!fn) return PERKS_ERR_UNSUPPORTED;
     int occ = 0;
     PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn),
                                                                kVariants[var].nw * 64, 0));
     perks_plan_info pl;
-    const int dtype = sizeof(T) == 8 ? PERKS_F64 : PERKS_F32;
     rc = plan_persistent(dtype, nz, ny, nx, o, &caps, occ, &pl);
     if (rc != PERKS_OK) return rc;
 
@@ -157,6 +165,7 @@ int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoe
     for (int l = 1; l <= kMaxLevels; ++l) a.nzs[l] = pl.reserved[l];
     a.sync = gs;
     a.timeout = grid_sync_timeout();
+    a.seed = o->round_seed;
     KCoefs kc = cf;
     void* args[] = {&d_a, &d_b, &a, &kc};
 
@@ -191,10 +200,27 @@ int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoe
     return PERKS_OK;
 }
 
+}  // namespace
+
+template <typename T>
+int run_persistent(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf, const perks_stencil_opts* o,
+                   hipStream_t stream, perks_run_report* rep)
+{
+    return run_persistent_fmt(sizeof(T) == 8 ? PERKS_F64 : PERKS_F32, d_a, d_b, nz, ny, nx, steps, cf, o, stream,
+                              rep);
+}
+
 template int run_persistent<double>(double*, double*, int, int, int, int, const KCoefs&, const perks_stencil_opts*,
                                     hipStream_t, perks_run_report*);
 template int run_persistent<float>(float*, float*, int, int, int, int, const KCoefs&, const perks_stencil_opts*,
                                    hipStream_t, perks_run_report*);
+
+int run_persistent16(int fmt, uint16_t* d_a, uint16_t* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
+                     const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep)
+{
+    if (fmt != PERKS_F16 && fmt != PERKS_BF16) return PERKS_ERR_INVALID_ARG;
+    return run_persistent_fmt(fmt, d_a, d_b, nz, ny, nx, steps, cf, o, stream, rep);
+}
 
 int release_persistent_scratch()
 {

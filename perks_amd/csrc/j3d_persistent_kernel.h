@@ -9,6 +9,7 @@
 
 #include "grid_sync.h"
 #include "j3d_common.h"
+#include "storage16.h"
 
 namespace perks {
 namespace j3dp {
@@ -22,6 +23,7 @@ struct PersistArgs {
     int nzs[kMaxLevels + 1];    // z-slabs per pass with l levels
     GridSync* sync;             // grid barrier state (grid_sync.h)
     unsigned long long timeout; // barrier wait limit, wall-clock ticks
+    uint32_t seed;              // stochastic rounding of 16-bit storage
 };
 
 template <typename T, int BC>
@@ -30,6 +32,57 @@ __device__ __forceinline__ T bc_value(T c)
     return BC == PERKS_BC_CLAMP ? c : (T)0;
 }
 
+// Storage format of the field: element type S in HBM, arithmetic type T.
+// FP16/BF16 values are widened on load and every level's result is rounded
+// to the storage format (the value a stored field would hold after that time
+// step), so a pass of L levels equals L stored steps bit for bit.
+template <int FMT>
+struct Elem;
+template <>
+struct Elem<PERKS_F64> {
+    using S = double;
+    using T = double;
+    __device__ static T load(rsrc_t r, uint32_t off) { return buf_load(r, off, double{}); }
+    __device__ static void store(rsrc_t r, uint32_t off, T v) { buf_store(r, off, v); }
+    __device__ static T q(T v) { return v; }
+    template <bool SR>
+    __device__ static T qr(T v, uint32_t, uint32_t) { return v; }
+};
+template <>
+struct Elem<PERKS_F32> {
+    using S = float;
+    using T = float;
+    __device__ static T load(rsrc_t r, uint32_t off) { return buf_load(r, off, float{}); }
+    __device__ static void store(rsrc_t r, uint32_t off, T v) { buf_store(r, off, v); }
+    __device__ static T q(T v) { return v; }
+    template <bool SR>
+    __device__ static T qr(T v, uint32_t, uint32_t) { return v; }
+};
+template <int FMT>
+struct Elem16 {
+    using S = uint16_t;
+    using T = float;
+    __device__ static T load(rsrc_t r, uint32_t off)
+    {
+        return Store16<FMT>::widen((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0));
+    }
+    __device__ static void store(rsrc_t r, uint32_t off, T v)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)Store16<FMT>::narrow(v), r, (int)off, 0, 0);
+    }
+    __device__ static T q(T v) { return round16<FMT>(v); }
+    // The stored value of cell idx at the step with hash key `key`.
+    template <bool SR>
+    __device__ static T qr(T v, uint32_t idx, uint32_t key)
+    {
+        return Store16<FMT>::widen(narrow_as<FMT, SR>(v, idx, key));
+    }
+};
+template <>
+struct Elem<PERKS_F16> : Elem16<PERKS_F16> {};
+template <>
+struct Elem<PERKS_BF16> : Elem16<PERKS_BF16> {};
+
 // One work item: L time steps over an x-y tile and a z-slab.
 //
 // Lanes 0 and 63 are ghost lanes at level 0 already (no halo column loads):
@@ -37,13 +90,16 @@ __device__ __forceinline__ T bc_value(T c)
 // one halo row on each side at level 0; from level 1 on, the rows that cross
 // between waves come from the LDS exchange, and the workgroup's outermost
 // rows are ghosts, so a tile stores VH = NW*NY - 2(L-1) rows.
-template <typename T, int BC, int NY, int NW, int P, int L, bool HAS_DIV>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
 struct March {
+    using E = Elem<FMT>;
+    using S = typename E::S;
+    using T = typename E::T;
     static constexpr int R = NY + 2;  // rows held per plane: the strip and one halo row each side
     static constexpr int D = P - 3;   // level-0 planes loaded ahead of the one level 1 needs next
     static constexpr int UF = P % 3 == 0 ? P : 3 * P;  // stages per loop trip (ring slots are static)
-    const T* __restrict__ src;
-    T* __restrict__ dst;
+    const S* __restrict__ src;
+    S* __restrict__ dst;
     T* xch;                           // LDS exchange rows [2][kMaxLevels-1][NW][2][64]
     int nz, ny, nx;
     size_t plane;
@@ -58,6 +114,10 @@ struct March {
     bool edge;  // this wave's strip touches an x or y face of the domain
     int lo[L + 1], hi[L + 1];
     int base;
+    // stochastic rounding: index of (y0, x) within a plane, cells per plane,
+    // and the hash key of each level's time step
+    uint32_t cbase, plane32;
+    uint32_t key[L + 1];
 
     T w0[P][R];
     T wl[L > 1 ? L - 1 : 1][3][R];
@@ -67,13 +127,13 @@ struct March {
         const int qc = q < nz ? q : nz - 1;
         const rsrc_t rs = make_rsrc(src + (size_t)qc * plane, plane_bytes);
 #pragma unroll
-        for (int r = 0; r < R; ++r) w0[slot][r] = buf_load(rs, roff[r], T{});
+        for (int r = 0; r < R; ++r) w0[slot][r] = E::load(rs, roff[r]);
     }
 
     // Level-l plane p from the three level-(l-1) planes b (p-1), c (p), u (p+1).
     // EDGE adds the boundary rule at domain faces; interior waves skip it.
     template <bool EDGE>
-    __device__ __forceinline__ void compute(const T (&b)[R], const T (&c)[R], const T (&u)[R], int p,
+    __device__ __forceinline__ void compute(const T (&b)[R], const T (&c)[R], const T (&u)[R], int p, int lv,
                                             T (&out)[NY]) const
     {
         const bool zlo = EDGE && p == 0;
@@ -93,15 +153,16 @@ struct March {
                 t = zhi ? bv : t;
                 bt = zlo ? bv : bt;
             }
-            out[r] = star1_update<T, HAS_DIV>(k, e, w, n, s, t, bt, cc);
+            const T v = star1_update<T, HAS_DIV>(k, e, w, n, s, t, bt, cc);
+            out[r] = E::template qr<SR>(v, (uint32_t)p * plane32 + cbase + (uint32_t)(r * nx), key[lv]);
         }
     }
 
-    __device__ __forceinline__ void level(const T (&b)[R], const T (&c)[R], const T (&u)[R], int p,
+    __device__ __forceinline__ void level(const T (&b)[R], const T (&c)[R], const T (&u)[R], int p, int lv,
                                           T (&out)[NY]) const
     {
-        if (edge || p == 0 || p == nz - 1) compute<true>(b, c, u, p, out);
-        else compute<false>(b, c, u, p, out);
+        if (edge || p == 0 || p == nz - 1) compute<true>(b, c, u, p, lv, out);
+        else compute<false>(b, c, u, p, lv, out);
     }
 
     // Publishes rows 0 and NY-1 of a level plane and gathers the rows just
@@ -124,7 +185,7 @@ struct March {
     {
         const rsrc_t rs = make_rsrc(dst + (size_t)p * plane, plane_bytes);
 #pragma unroll
-        for (int r = 0; r < NY; ++r) buf_store(rs, soff[r], out[r]);
+        for (int r = 0; r < NY; ++r) E::store(rs, soff[r], out[r]);
     }
 
     static constexpr int md(int a) { return ((a % 3) + 3) % 3; }
@@ -138,7 +199,7 @@ struct March {
             const int p = i - LV;
             if (p >= lo[LV] && p < hi[LV]) {
                 T out[NY];
-                level(wl[LV - 2][md(U - LV - 1)], wl[LV - 2][md(U - LV)], wl[LV - 2][md(U - LV + 1)], p, out);
+                level(wl[LV - 2][md(U - LV - 1)], wl[LV - 2][md(U - LV)], wl[LV - 2][md(U - LV + 1)], p, LV, out);
                 if constexpr (LV == L) store(p, out);
                 else exchange(LV, i & 1, out, wl[LV - 1][md(U - LV)]);
             }
@@ -155,7 +216,7 @@ struct March {
         const int p = i - 1;
         if (p >= lo[1] && p < hi[1]) {
             T out[NY];
-            level(w0[mp(U - 2)], w0[mp(U - 1)], w0[mp(U)], p, out);
+            level(w0[mp(U - 2)], w0[mp(U - 1)], w0[mp(U)], p, 1, out);
             if constexpr (L == 1) store(p, out);
             else exchange(1, i & 1, out, wl[0][md(U - 1)]);
         }
@@ -195,6 +256,7 @@ struct March {
         const int y0 = vy0 - (L - 1) + wave * NY;
         xlo = x == 0;
         xhi = x == nx - 1;
+        cbase = (uint32_t)y0 * (uint32_t)nx + (uint32_t)x;
         const bool xout = x >= vx0 && x < vx1;
         const bool xedge = xw0 <= 0 || xw0 + 63 >= nx - 1;
         const bool yedge = y0 <= 0 || y0 + NY - 1 >= ny - 1;
@@ -203,14 +265,14 @@ struct March {
         for (int r = 0; r < R; ++r) {
             const int yy = y0 + r - 1;
             const int yc = yy < 0 ? 0 : (yy >= ny ? ny - 1 : yy);
-            roff[r] = (uint32_t)((yc * nx + xc) * (int)sizeof(T));
+            roff[r] = (uint32_t)((yc * nx + xc) * (int)sizeof(S));
         }
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const int y = y0 + r;
             ylo[r] = y == 0;
             yhi[r] = y == ny - 1;
-            soff[r] = (xout && y >= vy0 && y < vy1) ? (uint32_t)((y * nx + x) * (int)sizeof(T)) : kOob;
+            soff[r] = (xout && y >= vy0 && y < vy1) ? (uint32_t)((y * nx + x) * (int)sizeof(S)) : kOob;
         }
 #pragma unroll
         for (int l = 0; l <= L; ++l) {
@@ -226,9 +288,10 @@ struct March {
     }
 };
 
-template <typename T, int BC, int NY, int NW, int P, int L, bool HAS_DIV>
-__device__ __forceinline__ void run_pass(const T* src, T* dst, T* xch, const PersistArgs& a, const T* k, int lane,
-                                         int wave)
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
+__device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typename Elem<FMT>::S* dst,
+                                         typename Elem<FMT>::T* xch, const PersistArgs& a,
+                                         const typename Elem<FMT>::T* k, int lane, int wave, int step0)
 {
     constexpr int VW = 64 - 2 * L;
     const int VH = NW * NY - 2 * (L - 1);
@@ -236,7 +299,7 @@ __device__ __forceinline__ void run_pass(const T* src, T* dst, T* xch, const Per
     const int G = gridDim.x;
     const int me = xcd_tile_id(blockIdx.x, G);
     for (int item = me; item < nitems; item += G) {
-        March<T, BC, NY, NW, P, L, HAS_DIV> m;
+        March<FMT, BC, NY, NW, P, L, HAS_DIV, SR> m;
         m.src = src;
         m.dst = dst;
         m.xch = xch;
@@ -244,19 +307,26 @@ __device__ __forceinline__ void run_pass(const T* src, T* dst, T* xch, const Per
         m.ny = a.ny;
         m.nx = a.nx;
         m.plane = (size_t)a.ny * a.nx;
-        m.plane_bytes = (uint32_t)(m.plane * sizeof(T));
+        m.plane_bytes = (uint32_t)(m.plane * sizeof(typename Elem<FMT>::S));
         m.lane = lane;
         m.wave = wave;
+        m.plane32 = (uint32_t)m.plane;
+        if constexpr (SR) {
+#pragma unroll
+            for (int l = 1; l <= L; ++l) m.key[l] = sr_key((uint32_t)(step0 + l - 1), a.seed);
+        }
 #pragma unroll
         for (int i = 0; i < 8; ++i) m.k[i] = k[i];
         m.run(item, a.nzs[L]);
     }
 }
 
-template <typename T, int BC, int NY, int NW, int P, bool HAS_DIV>
+template <int FMT, int BC, int NY, int NW, int P, bool HAS_DIV, bool SR>
 __global__ void __launch_bounds__(NW * 64)
-j3d_star1_persistent_kernel(T* buf_a, T* buf_b, PersistArgs a, KCoefs cf)
+j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S* buf_b, PersistArgs a, KCoefs cf)
 {
+    using S = typename Elem<FMT>::S;
+    using T = typename Elem<FMT>::T;
     __shared__ T xch[2 * (kMaxLevels - 1) * NW * 2 * 64];
     T k[8];
     load_coefs7<T>(cf, k);
@@ -264,17 +334,17 @@ j3d_star1_persistent_kernel(T* buf_a, T* buf_b, PersistArgs a, KCoefs cf)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = a.levels;
     const int npass = (a.steps + L - 1) / L;
-    T* src = buf_a;
-    T* dst = buf_b;
+    S* src = buf_a;
+    S* dst = buf_b;
     for (int pass = 0; pass < npass; ++pass) {
         const int lv = (pass == npass - 1) ? a.steps - pass * L : L;
         switch (lv) {
-        case 1: run_pass<T, BC, NY, NW, P, 1, HAS_DIV>(src, dst, xch, a, k, lane, wave); break;
-        case 2: run_pass<T, BC, NY, NW, P, 2, HAS_DIV>(src, dst, xch, a, k, lane, wave); break;
-        case 3: run_pass<T, BC, NY, NW, P, 3, HAS_DIV>(src, dst, xch, a, k, lane, wave); break;
-        default: run_pass<T, BC, NY, NW, P, 4, HAS_DIV>(src, dst, xch, a, k, lane, wave); break;
+        case 1: run_pass<FMT, BC, NY, NW, P, 1, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 2: run_pass<FMT, BC, NY, NW, P, 2, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 3: run_pass<FMT, BC, NY, NW, P, 3, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        default: run_pass<FMT, BC, NY, NW, P, 4, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
         }
-        T* t = src;
+        S* t = src;
         src = dst;
         dst = t;
         if (pass + 1 < npass && !grid_sync(a.sync, gridDim.x, a.timeout)) return;
@@ -290,27 +360,43 @@ struct Variant {
 constexpr Variant kVariants[] = {{4, 8, 6}, {4, 8, 9}};
 constexpr int kNumVariants = 2;
 
-template <typename T, int BC, bool HAS_DIV, int V>
+template <int FMT, int BC, bool HAS_DIV, int V, bool SR>
 void* kernel_ptr()
 {
     constexpr Variant v = kVariants[V];
-    return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<T, BC, v.ny, v.nw, v.p, HAS_DIV>);
+    return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, HAS_DIV, SR>);
 }
 
-template <typename T>
-void* pick_kernel(int bc, bool has_div, int var)
+template <int FMT, int BC, bool HAS_DIV, bool SR>
+void* pick_variant(int var)
 {
-    if (bc == PERKS_BC_ZERO) {
-        if (has_div) return var == 1 ? kernel_ptr<T, PERKS_BC_ZERO, true, 1>() : kernel_ptr<T, PERKS_BC_ZERO, true, 0>();
-        return var == 1 ? kernel_ptr<T, PERKS_BC_ZERO, false, 1>() : kernel_ptr<T, PERKS_BC_ZERO, false, 0>();
+    return var == 1 ? kernel_ptr<FMT, BC, HAS_DIV, 1, SR>() : kernel_ptr<FMT, BC, HAS_DIV, 0, SR>();
+}
+
+template <int FMT, bool SR>
+void* pick_bc(int bc, bool has_div, int var)
+{
+    if (bc == PERKS_BC_ZERO)
+        return has_div ? pick_variant<FMT, PERKS_BC_ZERO, true, SR>(var) : pick_variant<FMT, PERKS_BC_ZERO, false, SR>(var);
+    return has_div ? pick_variant<FMT, PERKS_BC_CLAMP, true, SR>(var) : pick_variant<FMT, PERKS_BC_CLAMP, false, SR>(var);
+}
+
+// sr: stochastic rounding (16-bit storage only; FP64/FP32 ignore it).
+template <int FMT>
+void* pick_kernel(int bc, bool has_div, int var, bool sr)
+{
+    if constexpr (FMT == PERKS_F16 || FMT == PERKS_BF16) {
+        if (sr) return pick_bc<FMT, true>(bc, has_div, var);
     }
-    if (has_div) return var == 1 ? kernel_ptr<T, PERKS_BC_CLAMP, true, 1>() : kernel_ptr<T, PERKS_BC_CLAMP, true, 0>();
-    return var == 1 ? kernel_ptr<T, PERKS_BC_CLAMP, false, 1>() : kernel_ptr<T, PERKS_BC_CLAMP, false, 0>();
+    return pick_bc<FMT, false>(bc, has_div, var);
 }
 
 }  // namespace j3dp
 
-void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant);
-void* j3d_persistent_kernel_f32(int bc, bool has_div, int variant);
+// Kernel entry points per storage format (j3d_persistent_<fmt>.hip).
+void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant, bool sr);
+void* j3d_persistent_kernel_f32(int bc, bool has_div, int variant, bool sr);
+void* j3d_persistent_kernel_f16(int bc, bool has_div, int variant, bool sr);
+void* j3d_persistent_kernel_bf16(int bc, bool has_div, int variant, bool sr);
 
 }  // namespace perks

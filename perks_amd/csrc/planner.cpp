@@ -44,7 +44,8 @@ extern "C" int perks_j3d_plan(int dtype, int height, int width_y, int width_x, c
                               const perks_device_caps* caps, int occupancy_hint, perks_plan_info* out)
 {
     if (!opts || !out || height <= 0 || width_y <= 0 || width_x <= 0) return PERKS_ERR_INVALID_ARG;
-    if (dtype != PERKS_F64 && dtype != PERKS_F32) return PERKS_ERR_UNSUPPORTED;
+    if (dtype != PERKS_F64 && dtype != PERKS_F32 && dtype != PERKS_F16 && dtype != PERKS_BF16)
+        return PERKS_ERR_UNSUPPORTED;
     perks_device_caps local;
     if (!caps) {
         int dev = 0;

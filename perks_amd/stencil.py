@@ -70,9 +70,12 @@ def shape_info(shape) -> ShapeInfo:
                      offsets)
 
 
+ROUNDING = {"nearest": 0, "stochastic": 1}
+
+
 def make_opts(shape="3d_star_r1", boundary="clamp", method="persistent", coefs=None, divisor=None,
               bdimx=0, blkpsm=0, use_lds_cache=0, reg_cache_planes=-1, double_tile=0, warmup=0,
-              warmup_steps=0, tune=None) -> L.StencilOpts:
+              warmup_steps=0, tune=None, rounding="nearest", round_seed=0) -> L.StencilOpts:
     lib = L.load()
     o = L.StencilOpts()
     lib.perks_stencil_opts_init(C.byref(o), shape_id(shape))
@@ -89,6 +92,8 @@ def make_opts(shape="3d_star_r1", boundary="clamp", method="persistent", coefs=N
     if tune:
         for i, v in enumerate(tune):
             o.tune[i] = int(v)
+    o.rounding = ROUNDING[rounding] if isinstance(rounding, str) else int(rounding)
+    o.round_seed = int(round_seed) & 0xFFFFFFFF
     return o
 
 
@@ -147,15 +152,15 @@ def _ptr(t) -> int:
 
 def j3d_device(d_a, d_b, height, width_y, width_x, steps, dtype="f64", stream=None, opts=None,
                **kw) -> L.RunReport:
-    """Runs on device-resident buffers (torch tensors or raw pointers).  The
-    result is in d_b when report.result_in_b, else in d_a."""
+    """Runs on device-resident buffers (torch tensors or raw pointers) of
+    f64 / f32 / f16 / bf16 storage (16-bit: raw bit patterns, FP32
+    arithmetic).  The result is in d_b when report.result_in_b, else in d_a."""
     lib = L.load()
     o = opts if opts is not None else make_opts(**kw)
     rep = L.RunReport()
-    fn = lib.perks_j3d_device_f64 if dtype in ("f64", L.F64) else lib.perks_j3d_device_f32
     s = None if stream is None else _ptr_stream(stream)
-    L.check(fn(_ptr(d_a), _ptr(d_b), height, width_y, width_x, int(steps), C.byref(o), s, C.byref(rep)),
-            "j3d_device")
+    L.check(lib.perks_j3d_device(_dt(dtype), _ptr(d_a), _ptr(d_b), height, width_y, width_x, int(steps), C.byref(o),
+                                 s, C.byref(rep)), "j3d_device")
     return rep
 
 

@@ -58,6 +58,11 @@ def lib() -> C.CDLL:
             "po_bf16_to_f32": (C.c_float, [C.c_uint16]),
             "po_stencil_iterate_f16s": (None, [P, P, I, I, I, P, I]),
             "po_stencil_iterate_bf16s": (None, [P, P, I, I, I, P, I]),
+            "po_lowbias32": (C.c_uint32, [C.c_uint32]),
+            "po_sr_bits": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+            "po_f32_to_bf16_sr": (C.c_uint16, [C.c_float, C.c_uint32]),
+            "po_f32_to_f16_sr": (C.c_uint16, [C.c_float, C.c_uint32]),
+            "po_stencil_iterate_sr16": (None, [P, P, I, I, I, P, I, I, C.c_uint32]),
             "po_rms_error_f64": (D, [P, P, I, I, I, I, I, I, I, I, P]),
             "po_rms_error_f32": (D, [P, P, I, I, I, I, I, I, I, I, P]),
             "po_poisson7_nnz": (C.c_int64, [I, I, I]),
@@ -148,6 +153,19 @@ def iterate16(field_bits: np.ndarray, st: PoStencil, steps: int, fmt: str) -> np
     out = np.empty_like(field_bits)
     fn = lib().po_stencil_iterate_f16s if fmt == "f16" else lib().po_stencil_iterate_bf16s
     fn(p(field_bits), p(out), nz, ny, nx, C.byref(st), steps)
+    return out
+
+
+def iterate16_sr(field_bits: np.ndarray, st: PoStencil, steps: int, fmt: str, seed: int) -> np.ndarray:
+    """As iterate16, with stochastic rounding keyed by (cell, step, seed)."""
+    field_bits = np.ascontiguousarray(field_bits, dtype=np.uint16)
+    if field_bits.ndim == 2:
+        nz, (ny, nx) = 1, field_bits.shape
+    else:
+        nz, ny, nx = field_bits.shape
+    out = np.empty_like(field_bits)
+    lib().po_stencil_iterate_sr16(p(field_bits), p(out), nz, ny, nx, C.byref(st), steps, 1 if fmt == "bf16" else 0,
+                                  seed & 0xFFFFFFFF)
     return out
 
 

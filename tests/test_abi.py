@@ -34,10 +34,32 @@ def test_ctypes_mirror_covers_the_header():
     assert set(declared_functions()) == set(L.SIGNATURES)
 
 
-def test_struct_sizes():
-    assert C.sizeof(L.StencilOpts) == 4 * 4 + 27 * 8 + 8 + 7 * 4 + 8 * 4 + 4  # trailing pad to 8
-    assert C.sizeof(L.RunReport) == 8 + 8 + 4 * (3 + 1 + 1 + 1 + 1 + 1 + 1 + 7)
-    assert C.sizeof(L.Grid2d) == 8 + 8 + 4 * 4
+STRUCTS = {"perks_stencil_opts": L.StencilOpts, "perks_run_report": L.RunReport, "perks_grid2d": L.Grid2d,
+           "perks_device_caps": L.DeviceCaps, "perks_plan_info": L.PlanInfo, "perks_csr": L.Csr, "perks_dia": L.Dia,
+           "perks_cg_opts": L.CgOpts, "perks_cg_report": L.CgReport}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof and every field offset of the ctypes mirror against the C compiler's."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "perks_amd.h"', 'int main(void) {']
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    seen = 0
+    for line in filter(None, out):
+        cname, what, value = line.split()
+        py = STRUCTS[cname]
+        want = C.sizeof(py) if what == "sizeof" else getattr(py, what).offset
+        assert int(value) == want, f"{cname}.{what}: C {value}, ctypes {want}"
+        seen += 1
+    assert seen == sum(len(p._fields_) + 1 for p in STRUCTS.values())
 
 
 def test_error_strings_and_version():

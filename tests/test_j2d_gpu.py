@@ -133,3 +133,26 @@ def test_2d_matches_committed_reference_gold(gpu, shape, tag, method):
     say(f"[j2d] reference gold vector {shape} {tag} {method} dims={field.shape} steps={steps}")
     got, _ = P.jacobi2d_run(field, steps, shape=shape, method=method, boundary="clamp")
     check_equal(got, GOLD[f"{shape}_{tag}_out"], f"{shape} {tag} {method} vs reference gold")
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+@pytest.mark.parametrize("shape", ["2d5pt", "2d25pt"])
+@pytest.mark.parametrize("method", ["plain", "persistent"])
+def test_2d_stochastic_rounding_matches_oracle(gpu, fmt, shape, method):
+    torch = gpu
+    dims, steps = (90, 200), 13
+    say(f"[j2d] stochastic rounding {fmt} {shape} {method}")
+    si = P.shape_info(shape)
+    rng = np.random.default_rng(8)
+    c = rng.random(len(si.coef))
+    coefs = list(c / c.sum())
+    n = dims[0] * dims[1]
+    a = torch.empty(n, dtype=torch.int16, device="cuda")
+    b = torch.empty(n, dtype=torch.int16, device="cuda")
+    P.fill_uniform(a, n, 22, -1.0, 1.0, dtype=fmt)
+    init = a.cpu().numpy().view(np.uint16).reshape(dims).copy()
+    rep = P.jacobi2d_device(a, b, dims[0], dims[1], steps, dtype=fmt, shape=shape, method=method, boundary="clamp",
+                            coefs=coefs, rounding="stochastic", round_seed=77)
+    got = (b if rep.result_in_b else a).cpu().numpy().view(np.uint16).reshape(dims)
+    want = O.iterate16_sr(init, O.stencil_from_shape(si, "clamp", coefs=coefs), steps, fmt, 77)
+    check_equal(got, want, f"{fmt} {shape} {method} stochastic")

@@ -302,3 +302,117 @@ def test_oracle_matches_committed_interior_gold(tag):
     f = GOLD[f"j3d_gold4_{tag}_in"]
     got = O.iterate(np.ascontiguousarray(f[1:-1, 1:-1, 1:-1]), O.stencil_from_shape(P.shape_info("3d7pt"), "zero"), 4)
     assert np.array_equal(got, GOLD[f"j3d_gold4_{tag}_out"][1:-1, 1:-1, 1:-1])
+
+
+# ---------------------------------------------------------------- stochastic rounding of 16-bit storage
+def test_lowbias32_is_a_bijection_with_full_avalanche():
+    f = O.lib().po_lowbias32
+    xs = np.arange(1 << 16, dtype=np.uint32) * np.uint32(40503)
+    hs = np.array([f(int(x)) for x in xs], dtype=np.uint32)
+    assert len(np.unique(hs)) == len(hs)
+    assert f(0) == 0
+    flips = [bin(f(int(x)) ^ f(int(x) ^ (1 << b))).count("1") for x in xs[:200] for b in range(32)]
+    assert 15.0 < np.mean(flips) < 17.0
+    # the per-step key changes every bit position about half the time
+    keys = np.array([O.lib().po_sr_bits(0, s, 7) for s in range(4096)], dtype=np.uint32)
+    bits = (keys[:, None] >> np.arange(32, dtype=np.uint32)) & 1
+    assert np.all(np.abs(bits.mean(axis=0) - 0.5) < 0.05)
+
+
+def _sr_mean(v, fmt):
+    """Mean of the stochastic rounding of v over every value of the random bits it uses."""
+    f = O.lib().po_f32_to_bf16_sr if fmt == "bf16" else O.lib().po_f32_to_f16_sr
+    nr = 1 << (16 if fmt == "bf16" else 13)
+    q = np.array([f(float(v), r) for r in range(nr)], dtype=np.uint16)
+    return O.from16(q, fmt).astype(np.float64), q
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "f16"])
+def test_stochastic_rounding_is_unbiased_and_brackets(fmt):
+    rng = np.random.default_rng(3)
+    vals = np.concatenate([rng.uniform(-2, 2, 12), rng.uniform(1e-3, 1e-2, 4), [0.1, -0.3, 1.0 / 3.0]]).astype(np.float32)
+    for v in vals:
+        w, q = _sr_mean(v, fmt)
+        # exactly unbiased: the mean over all random values is v itself
+        assert abs(w.mean() - float(v)) <= 1e-12 * max(1.0, abs(float(v))), (fmt, v)
+        # only the two neighbours of v in the format are produced
+        lo, hi = w.min(), w.max()
+        assert lo <= float(v) <= hi
+        assert len(np.unique(q)) <= 2
+    # representable values never move
+    for v in (0.5, -0.75, 1.0, 0.0, 2.0 ** -10):
+        w, _ = _sr_mean(np.float32(v), fmt)
+        assert np.all(w == v)
+
+
+def test_stochastic_rounding_specials():
+    f16, bf = O.lib().po_f32_to_f16_sr, O.lib().po_f32_to_bf16_sr
+    for r in (0, 1, 0x1FFF, 0xFFFF, 0xFFFFFFFF):
+        assert O.from16(np.array([bf(float("inf"), r)], np.uint16), "bf16")[0] == np.inf
+        assert np.isnan(O.from16(np.array([bf(float("nan"), r)], np.uint16), "bf16")[0])
+        assert O.from16(np.array([f16(float("-inf"), r)], np.uint16), "f16")[0] == -np.inf
+        # FP16 subnormal results round to nearest even whatever the bits
+        v = np.float32(3.3e-6)
+        assert f16(float(v), r) == O.lib().po_f32_to_f16(float(v))
+    # just above the largest FP16 value: 65504 or inf, in proportion
+    v = np.float32(65504.0 + 8.0)
+    w, _ = _sr_mean(v, "f16")
+    assert set(np.unique(w)) <= {65504.0, np.inf}
+    assert abs(np.mean(w == np.inf) - 0.25) < 1e-3
+
+
+def _numpy_lowbias32(x):
+    x = x.astype(np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7feb352d) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846ca68b) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x.astype(np.uint32)
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "f16"])
+def test_stochastic_iteration_matches_numpy_restatement(fmt):
+    """An independent numpy statement of the documented rounding rule (array
+    bit operations, its own hash) against the oracle's C loop."""
+    torch = pytest.importorskip("torch")
+    si = P.shape_info("3d7pt")
+    groups = [(CONFIG_3D[g], si.offsets[g], 0) for g in range(7)]
+    dims = (5, 6, 9)
+    seed = 1234
+    u = O.uniform(int(np.prod(dims)), 5, 0.0, 1.0, np.float32).reshape(dims)
+    bits = O.to16(u, fmt)
+    cur = O.from16(bits, fmt).reshape(dims)
+    idx = np.arange(int(np.prod(dims)), dtype=np.uint32).reshape(dims)
+    for s in range(4):
+        v = numpy_step(cur, groups, "zero").astype(np.float32)
+        key = _numpy_lowbias32(np.array([seed ^ ((s * 0x9E3779B9) & 0xFFFFFFFF)], dtype=np.uint64))[0]
+        r = _numpy_lowbias32(idx ^ key)
+        ub = v.view(np.uint32).astype(np.uint64)
+        if fmt == "bf16":
+            q = ((ub + (r & 0xFFFF)) >> 16).astype(np.uint16)
+            cur = O.from16(q, "bf16").reshape(dims)
+        else:
+            t = ((ub + (r & 0x1FFF)) & 0xFFFFE000).astype(np.uint32).view(np.float32)
+            e = (ub >> 23) & 0xFF
+            normal = (e >= 113) & (e < 255)
+            with np.errstate(over="ignore"):
+                rounded = np.where(normal, t, v).astype(np.float16)
+            cur = rounded.astype(np.float32)
+    got = O.from16(O.iterate16_sr(bits, O.stencil(groups, "zero"), 4, fmt, seed), fmt).reshape(dims)
+    assert np.array_equal(got, cur)
+
+
+def test_stochastic_rounding_removes_the_stall():
+    """Nearest rounding loses every change below half an ulp, so a decaying
+    BF16 field stalls; stochastic rounding follows the FP64 solution."""
+    si = P.shape_info("3d7pt")
+    dims, steps = (16, 16, 16), 300
+    st = O.stencil_from_shape(si, "zero", coefs=CONFIG_3D)
+    u = O.uniform(int(np.prod(dims)), 2, 0.0, 1.0).reshape(dims)
+    ref = O.iterate(u, st, steps)
+    bits = O.to16(u.astype(np.float32), "bf16")
+    rn = O.from16(O.iterate16(bits, st, steps, "bf16"), "bf16").reshape(dims)
+    sr = O.from16(O.iterate16_sr(bits, st, steps, "bf16", 99), "bf16").reshape(dims)
+    rel = lambda x: np.sqrt(np.mean((x - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    assert rel(sr) < 1e-2 < rel(rn)
