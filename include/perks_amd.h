@@ -99,7 +99,9 @@ typedef struct perks_stencil_opts {
     /* experimental knobs, 0 = automatic:
      *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 2 = cached stores (default nontemporal)
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
-     *   tune[4] persistent: tile variant (0..3)      tune[5..7] unused */
+     *   tune[4] persistent: tile variant (0..1)
+     *   tune[5] 3D: 1 = run the 7-point star on the shape-generic kernel
+     *   tune[6..7] unused */
     int32_t tune[8];
     /* FP16/BF16 storage only (no reference counterpart): how each step's FP32
      * result is rounded to the storage format.  PERKS_ROUND_NEAREST (default):

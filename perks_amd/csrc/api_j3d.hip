@@ -15,6 +15,7 @@
 #include "api_common.h"
 #include "j3d_kernels.h"
 #include "storage16.h"
+#include "j3d_general.h"
 #include "j3d_persistent.h"
 #include "planner.h"
 #include "j2d.h"
@@ -90,6 +91,13 @@ int run_plain(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const KCoefs& c
     return PERKS_OK;
 }
 
+// Every shape but the 7-point star runs on the shape-generic kernel; tune[5] = 1
+// sends the 7-point star there too.
+bool uses_general(const perks_stencil_opts* o)
+{
+    return o->shape != PERKS_SHAPE_3D_STAR_R1 || o->tune[5] == 1;
+}
+
 template <typename T>
 int validate(const T* a, const T* b, int nz, int ny, int nx, int steps, const perks_stencil_opts* o)
 {
@@ -103,7 +111,6 @@ int validate(const T* a, const T* b, int nz, int ny, int nx, int steps, const pe
     if (o->rounding != PERKS_ROUND_NEAREST && o->rounding != PERKS_ROUND_STOCHASTIC) return PERKS_ERR_INVALID_ARG;
     const ShapeDesc* s = shape_by_id(o->shape);
     if (!s || s->dims != 3) return PERKS_ERR_INVALID_ARG;
-    if (o->shape != PERKS_SHAPE_3D_STAR_R1) return PERKS_ERR_UNSUPPORTED;
     return PERKS_OK;
 }
 
@@ -118,6 +125,8 @@ int j3d_device(T* d_a, T* d_b, int nz, int ny, int nx, int steps, const perks_st
     if (rc != PERKS_OK) return rc;
     if (rep) std::memset(rep, 0, sizeof(*rep));
     hipStream_t s = (hipStream_t)stream;
+    if (uses_general(o))
+        return j3g_run(sizeof(T) == 8 ? PERKS_F64 : PERKS_F32, d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
     if (o->method == PERKS_METHOD_PLAIN) return run_plain<T>(d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
     return run_persistent<T>(d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
 }
@@ -177,6 +186,7 @@ int j3d_device16(int fmt, uint16_t* d_a, uint16_t* d_b, int nz, int ny, int nx, 
     if (rc != PERKS_OK) return rc;
     if (rep) std::memset(rep, 0, sizeof(*rep));
     hipStream_t s = (hipStream_t)stream;
+    if (uses_general(o)) return j3g_run(fmt, d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
     if (o->method == PERKS_METHOD_PLAIN) return run_plain16(fmt, d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
     return run_persistent16(fmt, d_a, d_b, nz, ny, nx, steps, cf, o, s, rep);
 }

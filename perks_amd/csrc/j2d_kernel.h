@@ -13,68 +13,14 @@
 #include <hip/hip_runtime.h>
 
 #include "j2d.h"
-#include "storage16.h"
+#include "storage_fmt.h"
 
 namespace perks {
 namespace j2dk {
 
-// ---- storage formats --------------------------------------------------------
-struct f16s {
-    uint16_t v;
-};
-struct bf16s {
-    uint16_t v;
-};
-
-template <typename S>
-struct Fmt;
-template <>
-struct Fmt<double> {
-    using C = double;
-    __device__ static C ld(const double* p, size_t i) { return p[i]; }
-    __device__ static void st(double* p, size_t i, C v) { p[i] = v; }
-    __device__ static C q(C v) { return v; }
-    __device__ static C qsr(C v, uint32_t, uint32_t) { return v; }
-};
-template <>
-struct Fmt<float> {
-    using C = float;
-    __device__ static C ld(const float* p, size_t i) { return p[i]; }
-    __device__ static void st(float* p, size_t i, C v) { p[i] = v; }
-    __device__ static C q(C v) { return v; }
-    __device__ static C qsr(C v, uint32_t, uint32_t) { return v; }
-};
-template <>
-struct Fmt<f16s> {
-    using C = float;
-    __device__ static C ld(const f16s* p, size_t i) { return __half2float(__ushort_as_half(p[i].v)); }
-    __device__ static void st(f16s* p, size_t i, C v) { p[i].v = __half_as_ushort(__float2half_rn(v)); }
-    // The value after a round trip through storage (every step is stored).
-    __device__ static C q(C v) { return __half2float(__float2half_rn(v)); }
-    // Stochastic rounding of cell idx at the step with hash key `key`.
-    __device__ static C qsr(C v, uint32_t idx, uint32_t key)
-    {
-        return Store16<PERKS_F16>::widen(narrow_as<PERKS_F16, true>(v, idx, key));
-    }
-};
-template <>
-struct Fmt<bf16s> {
-    using C = float;
-    __device__ static C ld(const bf16s* p, size_t i) { return __uint_as_float((uint32_t)p[i].v << 16); }
-    // Round to nearest even; NaN stays a quiet NaN.
-    __device__ static uint16_t bits(C v)
-    {
-        const uint32_t u = __float_as_uint(v);
-        return ((u & 0x7FFFFFFFu) > 0x7F800000u) ? (uint16_t)((u >> 16) | 0x40u)
-                                                 : (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-    }
-    __device__ static void st(bf16s* p, size_t i, C v) { p[i].v = bits(v); }
-    __device__ static C q(C v) { return __uint_as_float((uint32_t)bits(v) << 16); }
-    __device__ static C qsr(C v, uint32_t idx, uint32_t key)
-    {
-        return Store16<PERKS_BF16>::widen(narrow_as<PERKS_BF16, true>(v, idx, key));
-    }
-};
+using fmt::bf16s;
+using fmt::f16s;
+using fmt::Fmt;
 
 // ---- one cell update from the shape table ------------------------------------
 template <int SHAPE>

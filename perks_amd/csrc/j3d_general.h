@@ -1,0 +1,18 @@
+// j3d_general.h -- the shape-generic 3D Jacobi path (every 3D shape, every
+// storage format; per-step and persistent methods).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "j3d_common.h"
+
+namespace perks {
+
+// Runs `steps` time steps on device buffers d_a/d_b of format dtype; the
+// result is in d_b when rep->result_in_b.
+int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
+            const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);
+// Time steps per pass the persistent method uses for this shape and format.
+int j3g_levels(int dtype, int shape, const perks_stencil_opts* o);
+
+}  // namespace perks
