@@ -18,6 +18,8 @@ Other workloads (--workload; each prints its own JSON line):
   j3d-storage  the same 3D solve with FP64 / FP32 / FP16 / BF16 storage (FP32
                arithmetic for the 16-bit forms), persistent and per-step
                kernels, with max and RMS error against the FP64 result.
+  j3d-shapes   every 3D shape (7/13/17/27-point, Poisson) at 512^3, FP64 and
+               FP32, persistent and per-step kernels.
   j2d-storage  2D box r1 (2d9pt) and r2 (2d25pt), 8192^2, 500 steps, storage
                FP32 / FP16 / BF16 with FP32 arithmetic, against FP64 on the
                same inputs (max and RMS error at 10 / 100 / 500 steps).
@@ -57,7 +59,8 @@ def parse():
     ap.add_argument("--no-per-step", action="store_true", help="skip the per-step comparison")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
-    ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j2d-storage", "batch2d", "cg"], default="j3d")
+    ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j3d-shapes", "j2d-storage", "batch2d", "cg"],
+                    default="j3d")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
@@ -256,6 +259,53 @@ def run_j3d_storage(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "j3d_star1_plain16_kernel<bf16, stochastic> (x 1000 launches)",
                      "algorithmic_bytes_per_launch": alg},
+    }
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------- every 3D shape
+def run_j3d_shapes(args):
+    import torch
+    import perks_amd as P
+
+    n = args.size
+    T = 100 if args.tsteps == 1000 else args.tsteps
+    cells = n ** 3
+    rows = {}
+    for dt, tdt in (("f64", torch.float64), ("f32", torch.float32)):
+        a = torch.empty(cells, dtype=tdt, device="cuda")
+        b = torch.empty_like(a)
+        init = torch.empty_like(a)
+        P.fill_uniform(init, cells, args.seed, 0.0, 1.0, dtype=dt)
+        for shape in ("3d7pt", "3d13pt", "3d17pt", "3d27pt", "poisson"):
+            row = {}
+            for method in ("persistent", "plain"):
+                times = []
+                for _ in range(max(1, args.steps)):
+                    a.copy_(init)
+                    torch.cuda.synchronize()
+                    rep = P.j3d_device(a, b, n, n, n, T, dtype=dt, shape=shape, method=method, boundary="clamp")
+                    times.append(rep.elapsed_ms)
+                ms = sum(times) / len(times)
+                row[method] = {"gcells_per_s": round(T * cells / (ms / 1e3) / 1e9, 2), "ms_per_solve": round(ms, 3),
+                               "steps_per_pass": rep.reserved[0] if method == "persistent" else 1}
+            row["speedup"] = round(row["persistent"]["gcells_per_s"] / row["plain"]["gcells_per_s"], 3)
+            rows[f"{shape}_{dt}"] = row
+        del a, b, init
+        torch.cuda.empty_cache()
+    v = rows["3d27pt_f64"]["persistent"]
+    alg = T * cells * 2 * 8
+    achieved = alg / (v["ms_per_solve"] / 1e3) / 1e9
+    out = {
+        "metric": "GCells/s, every 3D shape at 512^3, persistent vs per-step", "value": v["gcells_per_s"],
+        "unit": "GCells/s", "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": v["ms_per_solve"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Jacobi, every reference shape with its reference weights, {n}^3, clamp boundary, "
+                               f"{T} time steps; value = 3d27pt FP64 persistent", "size": n, "time_steps": T,
+                   "results": rows},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "j3g_kernel<3d27pt, double>", "algorithmic_bytes_per_launch": alg},
     }
     print(json.dumps(out), flush=True)
 
@@ -517,8 +567,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
-    {"j3d": run_j3d, "j3d-storage": run_j3d_storage, "j2d-storage": run_j2d_storage, "batch2d": run_batch2d,
-     "cg": run_cg}[args.workload](args)
+    {"j3d": run_j3d, "j3d-storage": run_j3d_storage, "j3d-shapes": run_j3d_shapes, "j2d-storage": run_j2d_storage,
+     "batch2d": run_batch2d, "cg": run_cg}[args.workload](args)
 
 
 if __name__ == "__main__":

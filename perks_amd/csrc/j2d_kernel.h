@@ -94,7 +94,13 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
     }
     const double kdiv = cf.divisor;
 
-    const int tid = threadIdx.x;
+    // Lanes map to tile columns (lane, lane+64, ...), the four waves to rows
+    // w, w+4, w+8, ...: no per-cell index division.
+    constexpr int CX = TX / 64;
+    constexpr int kRowStep = 4;
+    static_assert(TX % 64 == 0, "tile width is a multiple of the wave");
+    const int lane = threadIdx.x & 63;
+    const int row0 = threadIdx.x >> 6;
     const int G = gridDim.x;
     const int me = xcd_tile_id(blockIdx.x, G);
 
@@ -126,11 +132,13 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
             const int nx = gd.nx, ny = gd.ny;
 
             __syncthreads();  // the previous item may still be reading the tile
-            for (int i = tid; i < TX * TY; i += 256) {
-                const int ly = i / TX, lx = i % TX;
-                const int y = min(max(oy + ly, 0), ny - 1);
-                const int x = min(max(ox + lx, 0), nx - 1);
-                tile[0][ly][lx] = Fmt<S>::ld(src, (size_t)y * nx + x);
+            for (int ly = row0; ly < TY; ly += kRowStep) {
+                const S* rowp = src + (size_t)min(max(oy + ly, 0), ny - 1) * nx;
+#pragma unroll
+                for (int cx = 0; cx < CX; ++cx) {
+                    const int lx = lane + 64 * cx;
+                    tile[0][ly][lx] = Fmt<S>::ld(rowp, (size_t)min(max(ox + lx, 0), nx - 1));
+                }
             }
             __syncthreads();
             // Domain edges in tile coordinates; tiles that reach past an edge
@@ -141,8 +149,12 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
                 const int cur = l & 1;
                 // stochastic rounding key of this level's time step (uniform)
                 const uint32_t key = a.sr ? sr_key((uint32_t)(gp * a.levels + l), a.seed) : 0u;
-                for (int i = tid; i < (TX - 2 * R) * (TY - 2 * R); i += 256) {
-                    const int ly = R + i / (TX - 2 * R), lx = R + i % (TX - 2 * R);
+                const int m = R * (l + 1);  // level l+1 is valid on the tile less m cells per side
+                for (int ly = m + row0; ly < TY - m; ly += kRowStep)
+#pragma unroll
+                for (int cx = 0; cx < CX; ++cx) {
+                    const int lx = lane + 64 * cx;
+                    if (lx < m || lx >= TX - m) continue;
                     C v;
                     if (!edge) {
                         v = eval_shape<SHAPE, C, HAS_DIV>(k, kdiv, kd, [&](int dy, int dx) {
@@ -171,10 +183,13 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
                 __syncthreads();
             }
             const int fin = L & 1;
-            for (int i = tid; i < VW * VH; i += 256) {
-                const int ly = RL + i / VW, lx = RL + i % VW;
-                const int y = oy + ly, x = ox + lx;
-                if (y < ny && x < nx) Fmt<S>::st(dst, (size_t)y * nx + x, tile[fin][ly][lx]);
+            for (int ly = RL + row0; ly < TY - RL && oy + ly < ny; ly += kRowStep) {
+                S* rowp = dst + (size_t)(oy + ly) * nx;
+#pragma unroll
+                for (int cx = 0; cx < CX; ++cx) {
+                    const int lx = lane + 64 * cx;
+                    if (lx >= RL && lx < TX - RL && ox + lx < nx) Fmt<S>::st(rowp, (size_t)(ox + lx), tile[fin][ly][lx]);
+                }
             }
         }
         if (pass + 1 < a.npass && !grid_sync(a.sync, G, a.timeout)) return;
