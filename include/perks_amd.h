@@ -227,6 +227,26 @@ typedef struct perks_grid2d {
 int perks_jacobi2d_batch(int dtype, perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts, void* stream,
                          perks_run_report* report);
 
+/* A reusable plan for a batch (no reference counterpart: the batched form is
+ * an addition of this library).  Creating it uploads the work list once;
+ * with autotune = 1 it times candidate launch configurations (tile shape,
+ * time steps per pass, workgroups per CU) on scratch buffers of the batch's
+ * sizes (at most 4 GiB of them, allocated and freed during the call) and keeps
+ * the fastest; then it captures the launch in a hipGraph.  Each
+ * perks_jacobi2d_batch_run replays the graph on `stream` and waits: every grid
+ * runs its steps from what its buffer a holds, as perks_jacobi2d_batch does.
+ * The buffers, sizes and step counts are fixed by the plan.  Two plans (or a
+ * plan and a direct batch call) must not run at the same time on one device.
+ * plan_info: result_in_b[i] per grid (ngrids entries, may be NULL) and
+ * config[6] = {tile variant, steps per pass, workgroups per CU, workgroups,
+ * autotuned, passes} (may be NULL). */
+typedef struct perks_batch_plan perks_batch_plan;
+int perks_jacobi2d_batch_plan(int dtype, const perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts,
+                              int autotune, void* stream, perks_batch_plan** plan);
+int perks_jacobi2d_batch_run(perks_batch_plan* plan, void* stream, perks_run_report* report);
+int perks_jacobi2d_batch_plan_info(const perks_batch_plan* plan, int32_t* result_in_b, int ngrids, int32_t* config);
+void perks_jacobi2d_batch_plan_destroy(perks_batch_plan* plan);
+
 /* ---- conjugate gradient ---------------------------------------------------- */
 /* Device-resident CSR: rowptr[n+1], col[nnz], val[nnz] (columns ascending
  * within a row, as the reference's CSR built by sparse_matrix.h:666-734). */

@@ -7,11 +7,11 @@ entry points; it loads the library and fails loudly if it is missing.
 from . import _lib
 from .cg import CgResult, cg_solve, poisson7_csr, poisson7_dia, poisson7_nnz
 from ._lib import PerksError, check, load
-from .stencil import (fill_uniform, j3d_device, j3d_iterative, j3d_run, jacobi2d_batch, jacobi2d_device,
+from .stencil import (BatchPlan, fill_uniform, j3d_device, j3d_iterative, j3d_run, jacobi2d_batch, jacobi2d_device,
                       jacobi2d_run, jacobi_iterative, make_opts, plan, query_device, shape_id, shape_info)
 
 __all__ = [
-    "CgResult", "cg_solve", "poisson7_csr", "poisson7_dia", "poisson7_nnz",
+    "BatchPlan", "CgResult", "cg_solve", "poisson7_csr", "poisson7_dia", "poisson7_nnz",
     "PerksError", "check", "load", "fill_uniform", "j3d_device", "j3d_iterative", "j3d_run", "jacobi2d_batch",
     "jacobi2d_device", "jacobi2d_run", "jacobi_iterative", "make_opts", "plan", "query_device", "shape_id",
     "shape_info", "_lib",

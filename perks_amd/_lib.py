@@ -139,6 +139,11 @@ SIGNATURES = {
     "perks_jacobi2d_iterative_f32": (_I, [_P, _I, _I, _P, _I, C.POINTER(StencilOpts), C.POINTER(RunReport)]),
     "perks_jacobi2d_device": (_I, [_I, _P, _P, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
     "perks_jacobi2d_batch": (_I, [_I, C.POINTER(Grid2d), _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
+    "perks_jacobi2d_batch_plan": (_I, [_I, C.POINTER(Grid2d), _I, C.POINTER(StencilOpts), _I, _P,
+                                       C.POINTER(C.c_void_p)]),
+    "perks_jacobi2d_batch_run": (_I, [C.c_void_p, _P, C.POINTER(RunReport)]),
+    "perks_jacobi2d_batch_plan_info": (_I, [C.c_void_p, C.POINTER(C.c_int32), _I, C.POINTER(C.c_int32)]),
+    "perks_jacobi2d_batch_plan_destroy": (None, [C.c_void_p]),
     "perks_cg_opts_init": (None, [C.POINTER(CgOpts)]),
     "perks_cg_solve_csr": (_I, [C.POINTER(Csr), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),
     "perks_cg_solve_dia": (_I, [C.POINTER(Dia), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),

@@ -45,9 +45,10 @@ int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o);
 int release_j2d_scratch();
 
 // Kernel entry points per storage type (j2d_<type>.hip); nullptr if not built.
-void* j2d_kernel_f64(int shape, int bc, bool has_div);
-void* j2d_kernel_f32(int shape, int bc, bool has_div);
-void* j2d_kernel_f16(int shape, int bc, bool has_div);
-void* j2d_kernel_bf16(int shape, int bc, bool has_div);
+// tile: 0 or 1 (TileOf in j2d_kernel.h).
+void* j2d_kernel_f64(int shape, int bc, bool has_div, int tile);
+void* j2d_kernel_f32(int shape, int bc, bool has_div, int tile);
+void* j2d_kernel_f16(int shape, int bc, bool has_div, int tile);
+void* j2d_kernel_bf16(int shape, int bc, bool has_div, int tile);
 
 }  // namespace perks

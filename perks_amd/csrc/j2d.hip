@@ -17,7 +17,11 @@
 //
 // The work list can span several independent grids (the batched form): each
 // grid has its own size, buffers and step count; grids are sorted by pass
-// count so the grids still active in a pass are a prefix of the list.
+// count so the grids still active in a pass are a prefix of the list.  A batch
+// plan (perks_jacobi2d_batch_plan) uploads the work list once, picks the tile
+// shape, the steps per pass and the workgroups per CU by timing candidates on
+// scratch buffers of the batch's sizes, and captures the launch in a hipGraph
+// that later calls replay.
 //
 // The kernel template itself is in j2d_kernel.h.
 #include <hip/hip_runtime.h>
@@ -34,49 +38,230 @@ namespace perks {
 
 namespace {
 
-void* kernel_for(int dtype, int shape, int bc, bool hd)
+void* kernel_for(int dtype, int shape, int bc, bool hd, int tile)
 {
     switch (dtype) {
-    case PERKS_F64: return j2d_kernel_f64(shape, bc, hd);
-    case PERKS_F32: return j2d_kernel_f32(shape, bc, hd);
-    case PERKS_F16: return j2d_kernel_f16(shape, bc, hd);
-    case PERKS_BF16: return j2d_kernel_bf16(shape, bc, hd);
+    case PERKS_F64: return j2d_kernel_f64(shape, bc, hd, tile);
+    case PERKS_F32: return j2d_kernel_f32(shape, bc, hd, tile);
+    case PERKS_F16: return j2d_kernel_f16(shape, bc, hd, tile);
+    case PERKS_BF16: return j2d_kernel_bf16(shape, bc, hd, tile);
     default: return nullptr;
     }
 }
 
-// Tile sizes of the instantiated kernels (j2d_kernel.h TileOf): 64 x 64 for
-// FP64, 128 x 64 for the FP32-arithmetic types; two LDS tiles of 32 KiB.
-void tile_dims(int dtype, int* tx, int* ty)
+// Tile sizes of the instantiated kernels (j2d_kernel.h TileOf).
+void tile_dims(int dtype, int tile, int* tx, int* ty)
 {
     *tx = dtype == PERKS_F64 ? 64 : 128;
-    *ty = 64;
+    *ty = tile == 1 ? 64 : 32;
 }
 
-std::mutex g_mu2;
-J2dGrid* g_grids[64] = {};
-int* g_active[64] = {};
-size_t g_grids_cap[64] = {}, g_active_cap[64] = {};
-
-int scratch2(int ngrids, int npass, J2dGrid** grids, int** active)
+int max_levels(int dtype, int tile, int radius)
 {
-    int dev = 0;
-    PERKS_HIP_TRY(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) return PERKS_ERR_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(g_mu2);
-    if (g_grids_cap[dev] < (size_t)ngrids) {
-        if (g_grids[dev]) PERKS_HIP_TRY(hipFree(g_grids[dev]));
-        PERKS_HIP_TRY(hipMalloc(&g_grids[dev], sizeof(J2dGrid) * (size_t)ngrids));
-        g_grids_cap[dev] = (size_t)ngrids;
+    int tx, ty;
+    tile_dims(dtype, tile, &tx, &ty);
+    return std::max(1, (std::min(tx, ty) - 2) / (2 * radius));  // at least 2 interior cells per side
+}
+
+int tile_of(const perks_stencil_opts* o) { return o->tune[4] == 1 ? 1 : 0; }
+
+// A launch configuration: tile variant, time steps per pass, workgroups per
+// CU (0: as many as fit).
+struct Config {
+    int tile;
+    int levels;
+    int occ;
+};
+
+// The host-side work list of a batch for a given pass depth.
+struct Work {
+    std::vector<J2dGrid> grids;  // sorted by pass count (descending); item_end cumulative
+    std::vector<int> active;     // grids still running in each pass (a prefix of `grids`)
+    std::vector<int> result_in_b;  // per input grid
+    int npass = 0;
+    int items = 0;
+    long long cell_steps = 0;
+};
+
+int build_work(const J2dSpec* specs, int n, int R, int tx, int ty, int L, Work* w)
+{
+    const int VW = tx - 2 * R * L, VH = ty - 2 * R * L;
+    if (VW < 1 || VH < 1) return PERKS_ERR_INVALID_ARG;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    auto npass_of = [&](int i) { return specs[i].steps > 0 ? (specs[i].steps + L - 1) / L : 0; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return npass_of(a) > npass_of(b); });
+    w->grids.clear();
+    w->result_in_b.assign(n, 0);
+    w->items = 0;
+    w->cell_steps = 0;
+    for (int i : order) {
+        const J2dSpec& s = specs[i];
+        if (s.ny <= 0 || s.nx <= 0 || s.steps < 0 || !s.a || !s.b) return PERKS_ERR_INVALID_ARG;
+        w->result_in_b[i] = npass_of(i) % 2;
+        if (s.steps == 0) continue;
+        // Tiles are laid out with the full-pass geometry; a shorter last pass
+        // reuses them (a smaller ghost rim is enough for it).
+        const long long tiles = (long long)((s.nx + VW - 1) / VW) * ((s.ny + VH - 1) / VH);
+        if (w->items + tiles > 0x7fffffffLL) return PERKS_ERR_UNSUPPORTED_SIZE;
+        J2dGrid g;
+        g.a = s.a;
+        g.b = s.b;
+        g.ny = s.ny;
+        g.nx = s.nx;
+        g.steps = s.steps;
+        g.tiles = (int)tiles;
+        w->items += g.tiles;
+        g.item_end = w->items;
+        w->grids.push_back(g);
+        w->cell_steps += (long long)s.steps * s.ny * s.nx;
     }
-    if (g_active_cap[dev] < (size_t)npass) {
-        if (g_active[dev]) PERKS_HIP_TRY(hipFree(g_active[dev]));
-        PERKS_HIP_TRY(hipMalloc(&g_active[dev], sizeof(int) * (size_t)npass));
-        g_active_cap[dev] = (size_t)npass;
+    w->npass = w->grids.empty() ? 0 : (w->grids[0].steps + L - 1) / L;
+    w->active.assign(w->npass, 0);
+    for (int p = 0; p < w->npass; ++p) {
+        int c = 0;
+        while (c < (int)w->grids.size() && (w->grids[c].steps + L - 1) / L > p) ++c;
+        w->active[p] = c;
     }
-    *grids = g_grids[dev];
-    *active = g_active[dev];
     return PERKS_OK;
+}
+
+// Device copies of a work list.
+struct DeviceWork {
+    J2dGrid* grids = nullptr;
+    int* active = nullptr;
+    size_t grids_cap = 0, active_cap = 0;
+
+    int upload(const Work& w, hipStream_t stream)
+    {
+        if (grids_cap < w.grids.size()) {
+            if (grids) PERKS_HIP_TRY(hipFree(grids));
+            PERKS_HIP_TRY(hipMalloc(&grids, sizeof(J2dGrid) * w.grids.size()));
+            grids_cap = w.grids.size();
+        }
+        if (active_cap < w.active.size()) {
+            if (active) PERKS_HIP_TRY(hipFree(active));
+            PERKS_HIP_TRY(hipMalloc(&active, sizeof(int) * w.active.size()));
+            active_cap = w.active.size();
+        }
+        PERKS_HIP_TRY(hipMemcpyAsync(grids, w.grids.data(), sizeof(J2dGrid) * w.grids.size(), hipMemcpyHostToDevice,
+                                     stream));
+        PERKS_HIP_TRY(hipMemcpyAsync(active, w.active.data(), sizeof(int) * w.active.size(), hipMemcpyHostToDevice,
+                                     stream));
+        return PERKS_OK;
+    }
+    int release()
+    {
+        if (grids) PERKS_HIP_TRY(hipFree(grids));
+        if (active) PERKS_HIP_TRY(hipFree(active));
+        grids = nullptr;
+        active = nullptr;
+        grids_cap = active_cap = 0;
+        return PERKS_OK;
+    }
+};
+
+std::mutex g_mu2;
+DeviceWork g_work[64];  // per device, for the direct (unplanned) calls
+
+// Everything a launch needs once the work list is on the device.
+struct Launch {
+    void* fn = nullptr;
+    int grid = 0;     // persistent: workgroups; plain: unused (per-step grids from the work list)
+    int occ = 0;
+    J2dArgs args;
+    KCoefs cf;
+};
+
+int prepare_launch(int dtype, const perks_stencil_opts* o, const KCoefs& cf, const Config& c, const Work& w,
+                   const DeviceWork& dw, GridSync* gs, Launch* out)
+{
+    perks_device_caps caps;
+    int rc = device_caps_cached(&caps);
+    if (rc != PERKS_OK) return rc;
+    out->fn = kernel_for(dtype, o->shape, o->boundary, cf.divisor != 0.0, c.tile);
+    if (!out->fn) return PERKS_ERR_UNSUPPORTED;
+    std::memset(&out->args, 0, sizeof(out->args));
+    out->args.grids = dw.grids;
+    out->args.active = dw.active;
+    out->args.levels = c.levels;
+    out->args.sync = gs;
+    out->args.timeout = grid_sync_timeout();
+    out->args.sr = (dtype == PERKS_F16 || dtype == PERKS_BF16) && o->rounding == PERKS_ROUND_STOCHASTIC;
+    out->args.seed = o->round_seed;
+    out->cf = cf;
+    out->occ = 0;
+    out->grid = 0;
+    if (o->method != PERKS_METHOD_PLAIN) {
+        int occ = 0;
+        PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(out->fn), 256, 0));
+        if (occ <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
+        if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
+        if (c.occ > 0 && c.occ < occ) occ = c.occ;
+        out->occ = occ;
+        out->grid = std::min(caps.cu_count * occ, w.items);
+        out->args.npass = w.npass;
+        out->args.step_offset = 0;
+    }
+    return PERKS_OK;
+}
+
+// Enqueues the launch(es) on `stream`.  cooperative: hipLaunchCooperativeKernel
+// (checked co-residency); otherwise an ordinary launch of the same grid, which
+// a stream capture can record (the grid is at most CUs x occupancy, and the
+// barrier's bounded wait turns a grid that is not co-resident into
+// PERKS_ERR_BARRIER_TIMEOUT instead of a hang).
+int enqueue(Launch& l, const perks_stencil_opts* o, const Work& w, hipStream_t stream, bool cooperative)
+{
+    if (w.grids.empty()) return PERKS_OK;
+    if (o->method == PERKS_METHOD_PLAIN) {
+        // One launch per step; each launch is a single pass over every tile.
+        const int steps = w.grids[0].steps;
+        J2dArgs a = l.args;
+        a.npass = 1;
+        for (int s = 0; s < steps; ++s) {
+            a.active = l.args.active + s;
+            a.step_offset = s;
+            const int g = w.grids[w.active[s] - 1].item_end;
+            void* args[] = {&a, &l.cf};
+            PERKS_HIP_TRY(hipLaunchKernel(l.fn, dim3(g), dim3(256), args, 0, stream));
+        }
+        return PERKS_OK;
+    }
+    void* args[] = {&l.args, &l.cf};
+    if (cooperative) PERKS_HIP_TRY(hipLaunchCooperativeKernel(l.fn, dim3(l.grid), dim3(256), args, 0, stream));
+    else PERKS_HIP_TRY(hipLaunchKernel(l.fn, dim3(l.grid), dim3(256), args, 0, stream));
+    return PERKS_OK;
+}
+
+int check_opts(const perks_stencil_opts* o, const ShapeDesc** sd_out)
+{
+    const ShapeDesc* sd = shape_by_id(o->shape);
+    if (!sd || sd->dims != 2) return PERKS_ERR_INVALID_ARG;
+    if (o->boundary != PERKS_BC_CLAMP && o->boundary != PERKS_BC_ZERO) return PERKS_ERR_INVALID_ARG;
+    if (o->method != PERKS_METHOD_PLAIN && o->method != PERKS_METHOD_PERSISTENT) return PERKS_ERR_INVALID_ARG;
+    if (o->rounding != PERKS_ROUND_NEAREST && o->rounding != PERKS_ROUND_STOCHASTIC) return PERKS_ERR_INVALID_ARG;
+    *sd_out = sd;
+    return PERKS_OK;
+}
+
+void fill_report(perks_run_report* rep, float ms, const Work& w, const Launch& l, const Config& c, int dtype)
+{
+    if (!rep) return;
+    int tx, ty;
+    tile_dims(dtype, c.tile, &tx, &ty);
+    std::memset(rep, 0, sizeof(*rep));
+    rep->elapsed_ms = ms;
+    rep->gcells_per_s = ms > 0 ? (double)w.cell_steps / ms / 1e6 : 0.0;
+    rep->grid[0] = l.grid > 0 ? l.grid : (w.grids.empty() ? 0 : w.grids.back().item_end);
+    rep->grid[1] = rep->grid[2] = 1;
+    rep->block = 256;
+    rep->lds_bytes = 2 * tx * ty * (dtype == PERKS_F64 ? 8 : 4);
+    rep->occupancy = l.occ;
+    rep->result_in_b = w.result_in_b.empty() ? 0 : w.result_in_b[0];
+    rep->reserved[0] = c.levels;
+    rep->reserved[1] = w.npass;
+    rep->reserved[2] = c.tile;
 }
 
 }  // namespace
@@ -85,157 +270,67 @@ int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o)
 {
     const ShapeDesc* sd = shape_by_id(shape);
     if (!sd) return 1;
-    int tx, ty;
-    tile_dims(dtype, &tx, &ty);
     if (o->method == PERKS_METHOD_PLAIN) return 1;
-    int L = o->tune[2] > 0 ? o->tune[2] : 0;
-    if (L <= 0) {
-        // Default: ghost rims of about a quarter of the shorter tile side.
-        L = std::max(1, (std::min(tx, ty) / 4) / sd->radius);
-        L = std::min(L, 8);
-    }
-    const int maxL = (std::min(tx, ty) - 2) / (2 * sd->radius);  // at least 2 interior cells per side
-    return std::max(1, std::min(L, maxL));
+    const int tile = tile_of(o);
+    int tx, ty;
+    tile_dims(dtype, tile, &tx, &ty);
+    // Default: 4 / R steps per pass (the best depths measured at 8192^2 for
+    // radius 1, 2 and 3-6 on the default tile).
+    const int L = o->tune[2] > 0 ? o->tune[2] : std::max(1, 4 / sd->radius);
+    return std::max(1, std::min(L, max_levels(dtype, tile, sd->radius)));
 }
 
 int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream,
             perks_run_report* rep)
 {
     if (!o || !specs || n <= 0) return PERKS_ERR_INVALID_ARG;
-    const ShapeDesc* sd = shape_by_id(o->shape);
-    if (!sd || sd->dims != 2) return PERKS_ERR_INVALID_ARG;
-    if (o->boundary != PERKS_BC_CLAMP && o->boundary != PERKS_BC_ZERO) return PERKS_ERR_INVALID_ARG;
-    if (o->rounding != PERKS_ROUND_NEAREST && o->rounding != PERKS_ROUND_STOCHASTIC) return PERKS_ERR_INVALID_ARG;
+    const ShapeDesc* sd = nullptr;
+    int rc = check_opts(o, &sd);
+    if (rc != PERKS_OK) return rc;
     KCoefs cf;
-    int rc = resolve_coefs(o, &cf, nullptr);
+    rc = resolve_coefs(o, &cf, nullptr);
     if (rc != PERKS_OK) return rc;
-    perks_device_caps caps;
-    rc = device_caps_cached(&caps);
-    if (rc != PERKS_OK) return rc;
-
+    const Config c{tile_of(o), j2d_pass_levels(dtype, o->shape, o), o->tune[3]};
     int tx, ty;
-    tile_dims(dtype, &tx, &ty);
-    const int L = j2d_pass_levels(dtype, o->shape, o);
-    const int VW = tx - 2 * sd->radius * L, VH = ty - 2 * sd->radius * L;
-
-    // Sort by pass count (descending) so the grids active in a pass are a prefix.
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    auto npass_of = [&](int i) { return specs[i].steps > 0 ? (specs[i].steps + L - 1) / L : 0; };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return npass_of(a) > npass_of(b); });
-    std::vector<J2dGrid> grids;
-    grids.reserve(n);
-    int items = 0;
-    long long cells_steps = 0;
-    for (int i : order) {
-        const J2dSpec& s = specs[i];
-        if (s.ny <= 0 || s.nx <= 0 || s.steps < 0 || !s.a || !s.b) return PERKS_ERR_INVALID_ARG;
-        if (s.steps == 0) continue;
-        // Tiles are laid out with the full-pass geometry; a shorter last pass
-        // reuses them (a smaller ghost rim is enough for it).
-        const int ntx = (s.nx + VW - 1) / VW, nty = (s.ny + VH - 1) / VH;
-        J2dGrid g;
-        g.a = s.a;
-        g.b = s.b;
-        g.ny = s.ny;
-        g.nx = s.nx;
-        g.steps = s.steps;
-        g.tiles = ntx * nty;
-        items += g.tiles;
-        g.item_end = items;
-        grids.push_back(g);
-        cells_steps += (long long)s.steps * s.ny * s.nx;
-    }
-    for (int i = 0; i < n; ++i) specs[i].result_in_b = npass_of(i) % 2;
-    if (grids.empty()) {
+    tile_dims(dtype, c.tile, &tx, &ty);
+    Work w;
+    rc = build_work(specs, n, sd->radius, tx, ty, c.levels, &w);
+    if (rc != PERKS_OK) return rc;
+    for (int i = 0; i < n; ++i) specs[i].result_in_b = w.result_in_b[i];
+    if (w.grids.empty()) {
         if (rep) std::memset(rep, 0, sizeof(*rep));
         return PERKS_OK;
     }
-    const int npass = (grids[0].steps + L - 1) / L;
-    std::vector<int> active(npass);
-    for (int p = 0; p < npass; ++p) {
-        int c = 0;
-        while (c < (int)grids.size() && (grids[c].steps + L - 1) / L > p) ++c;
-        active[p] = c;
-    }
-    // item_end is cumulative over the sorted list, so a prefix of grids covers
-    // a prefix of items.  Tiles per grid do not depend on the pass.
-    J2dGrid* d_grids = nullptr;
-    int* d_active = nullptr;
-    rc = scratch2((int)grids.size(), npass, &d_grids, &d_active);
+    int dev = 0;
+    PERKS_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return PERKS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g_mu2);
+    rc = g_work[dev].upload(w, stream);
     if (rc != PERKS_OK) return rc;
     GridSync* gs = nullptr;
     rc = grid_sync_get(kSyncJ2d, &gs);
     if (rc != PERKS_OK) return rc;
-    PERKS_HIP_TRY(hipMemcpyAsync(d_grids, grids.data(), sizeof(J2dGrid) * grids.size(), hipMemcpyHostToDevice,
-                                 stream));
-    PERKS_HIP_TRY(hipMemcpyAsync(d_active, active.data(), sizeof(int) * npass, hipMemcpyHostToDevice, stream));
-
-    void* fn = kernel_for(dtype, o->shape, o->boundary, cf.divisor != 0.0);
-    if (!fn) return PERKS_ERR_UNSUPPORTED;
-    J2dArgs a;
-    a.grids = d_grids;
-    a.active = d_active;
-    a.levels = L;
-    a.sync = gs;
-    a.timeout = grid_sync_timeout();
-    a.sr = (dtype == PERKS_F16 || dtype == PERKS_BF16) && o->rounding == PERKS_ROUND_STOCHASTIC;
-    a.seed = o->round_seed;
+    Launch l;
+    rc = prepare_launch(dtype, o, cf, c, w, g_work[dev], gs, &l);
+    if (rc != PERKS_OK) return rc;
 
     hipEvent_t ev0, ev1;
     PERKS_HIP_TRY(hipEventCreate(&ev0));
     PERKS_HIP_TRY(hipEventCreate(&ev1));
-    int grid_launched = 0, occ_used = 0;
     PERKS_HIP_TRY(hipEventRecord(ev0, stream));
-    if (o->method == PERKS_METHOD_PLAIN) {
-        // One launch per step; each launch is a single pass over every tile.
-        const int steps = grids[0].steps;
-        for (int s = 0; s < steps; ++s) {
-            // Per-step launches use the single-pass geometry of pass s.
-            a.npass = 1;
-            a.active = d_active + s;
-            a.step_offset = s;
-            const int gi = active[s] - 1;
-            grid_launched = grids[gi].item_end;
-            void* args[] = {&a, &cf};
-            PERKS_HIP_TRY(hipLaunchKernel(fn, dim3(grid_launched), dim3(256), args, 0, stream));
-        }
-    } else {
-        int occ = 0;
-        PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn), 256, 0));
-        if (occ <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
-        if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
-        if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
-        occ_used = occ;
-        grid_launched = std::min(caps.cu_count * occ, items);
-        a.npass = npass;
-        a.step_offset = 0;
-        void* args[] = {&a, &cf};
-        PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid_launched), dim3(256), args, 0, stream));
-    }
+    rc = enqueue(l, o, w, stream, true);
     PERKS_HIP_TRY(hipEventRecord(ev1, stream));
     PERKS_HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0.f;
     PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
+    if (rc != PERKS_OK) return rc;
     if (o->method != PERKS_METHOD_PLAIN) {
         rc = grid_sync_check(gs, stream);
         if (rc != PERKS_OK) return rc;
     }
-    if (rep) {
-        std::memset(rep, 0, sizeof(*rep));
-        rep->elapsed_ms = ms;
-        rep->gcells_per_s = ms > 0 ? (double)cells_steps / ms / 1e6 : 0.0;
-        rep->grid[0] = grid_launched;
-        rep->grid[1] = rep->grid[2] = 1;
-        rep->block = 256;
-        rep->lds_bytes = 2 * tx * ty * (dtype == PERKS_F64 ? 8 : 4);
-        rep->occupancy = occ_used;
-        rep->result_in_b = specs[0].result_in_b;
-        rep->reserved[0] = L;
-        rep->reserved[1] = npass;
-    }
+    fill_report(rep, ms, w, l, c, dtype);
     return PERKS_OK;
 }
 
@@ -245,12 +340,246 @@ int release_j2d_scratch()
     PERKS_HIP_TRY(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_mu2);
     if (dev < 0 || dev >= 64) return PERKS_OK;
-    if (g_grids[dev]) PERKS_HIP_TRY(hipFree(g_grids[dev]));
-    if (g_active[dev]) PERKS_HIP_TRY(hipFree(g_active[dev]));
-    g_grids[dev] = nullptr;
-    g_active[dev] = nullptr;
-    g_grids_cap[dev] = g_active_cap[dev] = 0;
-    return PERKS_OK;
+    return g_work[dev].release();
 }
 
 }  // namespace perks
+
+// ---- batch plans ---------------------------------------------------------------
+struct perks_batch_plan {
+    int dtype = 0;
+    int device = 0;
+    perks_stencil_opts opts;
+    perks::Config cfg{0, 1, 0};
+    perks::Work work;
+    perks::DeviceWork dw;
+    perks::Launch launch;
+    perks::GridSync* gs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int tuned = 0;
+    double tuned_gcells = 0.0;
+};
+
+namespace perks {
+namespace {
+
+// Times one configuration on scratch buffers: every grid runs min(steps, cap)
+// steps.  Returns cells per second (0 when the configuration does not apply).
+double time_config(int dtype, const perks_stencil_opts* o, const KCoefs& cf, const Config& c,
+                   const std::vector<J2dSpec>& scratch, int R, int cap, hipStream_t stream, GridSync* gs)
+{
+    int tx, ty;
+    tile_dims(dtype, c.tile, &tx, &ty);
+    if (c.levels > max_levels(dtype, c.tile, R)) return 0.0;
+    std::vector<J2dSpec> specs = scratch;
+    for (auto& s : specs) s.steps = std::min(s.steps, cap);
+    Work w;
+    if (build_work(specs.data(), (int)specs.size(), R, tx, ty, c.levels, &w) != PERKS_OK || w.grids.empty())
+        return 0.0;
+    DeviceWork dw;
+    double best = 0.0;
+    do {
+        if (dw.upload(w, stream) != PERKS_OK) break;
+        Launch l;
+        if (prepare_launch(dtype, o, cf, c, w, dw, gs, &l) != PERKS_OK) break;
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess) break;
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            break;
+        }
+        for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
+            if (hipEventRecord(e0, stream) != hipSuccess) break;
+            if (enqueue(l, o, w, stream, true) != PERKS_OK) break;
+            if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
+            if (o->method != PERKS_METHOD_PLAIN && grid_sync_check(gs, stream) != PERKS_OK) break;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+            if (rep == 1 && ms > 0) best = (double)w.cell_steps / (ms * 1e-3);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } while (0);
+    (void)dw.release();
+    return best;
+}
+
+constexpr size_t kTuneScratchBytes = 4ull << 30;  // scratch buffers the autotuner may allocate
+constexpr int kTuneSteps = 48;                    // time steps per grid in a trial (multiple of 1,2,3,4,6,8,12,16)
+
+int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_stencil_opts* o, int autotune,
+               hipStream_t stream, perks_batch_plan** out)
+{
+    const ShapeDesc* sd = nullptr;
+    int rc = check_opts(o, &sd);
+    if (rc != PERKS_OK) return rc;
+    KCoefs cf;
+    rc = resolve_coefs(o, &cf, nullptr);
+    if (rc != PERKS_OK) return rc;
+    std::vector<J2dSpec> specs((size_t)ngrids);
+    for (int i = 0; i < ngrids; ++i) {
+        if (!grids[i].a || !grids[i].b || grids[i].width_y <= 0 || grids[i].width_x <= 0 || grids[i].steps < 0)
+            return PERKS_ERR_INVALID_ARG;
+        specs[i] = J2dSpec{grids[i].a, grids[i].b, grids[i].width_y, grids[i].width_x, grids[i].steps, 0};
+    }
+    perks_batch_plan* p = new perks_batch_plan();
+    p->dtype = dtype;
+    p->opts = *o;
+    PERKS_HIP_TRY(hipGetDevice(&p->device));
+    rc = grid_sync_get(kSyncJ2d, &p->gs);
+    if (rc != PERKS_OK) {
+        delete p;
+        return rc;
+    }
+    p->cfg = Config{tile_of(o), j2d_pass_levels(dtype, o->shape, o), o->tune[3]};
+
+    if (autotune && o->method != PERKS_METHOD_PLAIN) {
+        // Scratch copies of (a prefix of) the batch's grids; contents do not matter for timing.
+        const size_t esz = dtype == PERKS_F64 ? 8 : (dtype == PERKS_F32 ? 4 : 2);
+        std::vector<J2dSpec> scratch;
+        std::vector<void*> bufs;
+        size_t used = 0;
+        for (const J2dSpec& s : specs) {
+            const size_t bytes = (size_t)s.ny * s.nx * esz;
+            if (used + 2 * bytes > kTuneScratchBytes) break;
+            void *a = nullptr, *b = nullptr;
+            if (hipMalloc(&a, bytes) != hipSuccess) break;
+            if (hipMalloc(&b, bytes) != hipSuccess) {
+                (void)hipFree(a);
+                break;
+            }
+            (void)hipMemsetAsync(a, 0, bytes, stream);
+            (void)hipMemsetAsync(b, 0, bytes, stream);
+            bufs.push_back(a);
+            bufs.push_back(b);
+            scratch.push_back(J2dSpec{a, b, s.ny, s.nx, s.steps, 0});
+            used += 2 * bytes;
+        }
+        (void)hipStreamSynchronize(stream);
+        double best = 0.0;
+        Config best_c = p->cfg;
+        const int occ_choices[] = {0, 1};
+        const int level_choices[] = {1, 2, 3, 4, 6, 8, 12, 16};
+        for (int tile = 0; tile <= 1 && !scratch.empty(); ++tile)
+            for (int L : level_choices)
+                for (int occ : occ_choices) {
+                    const Config c{tile, L, occ};
+                    const double v = time_config(dtype, o, cf, c, scratch, sd->radius, kTuneSteps, stream, p->gs);
+                    if (v > best) {
+                        best = v;
+                        best_c = c;
+                    }
+                }
+        for (void* b : bufs) (void)hipFree(b);
+        if (best > 0.0) {
+            p->cfg = best_c;
+            p->tuned = 1;
+            p->tuned_gcells = best / 1e9;
+        }
+    }
+
+    int tx, ty;
+    tile_dims(dtype, p->cfg.tile, &tx, &ty);
+    rc = build_work(specs.data(), ngrids, sd->radius, tx, ty, p->cfg.levels, &p->work);
+    if (rc == PERKS_OK) rc = p->dw.upload(p->work, stream);
+    if (rc == PERKS_OK) rc = prepare_launch(dtype, o, cf, p->cfg, p->work, p->dw, p->gs, &p->launch);
+    if (rc == PERKS_OK && hipStreamSynchronize(stream) != hipSuccess) rc = PERKS_ERR_HIP;
+    if (rc == PERKS_OK && !p->work.grids.empty()) {
+        // Capture the launch(es) on a stream of our own (default priority).
+        hipStream_t cap = nullptr;
+        if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) rc = PERKS_ERR_HIP;
+        if (rc == PERKS_OK) {
+            if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = PERKS_ERR_HIP;
+            const int erc = rc == PERKS_OK ? enqueue(p->launch, o, p->work, cap, false) : rc;
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(cap, &g);
+            if (rc == PERKS_OK && (erc != PERKS_OK || ec != hipSuccess)) rc = erc != PERKS_OK ? erc : PERKS_ERR_HIP;
+            if (rc == PERKS_OK) {
+                p->graph = g;
+                if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) rc = PERKS_ERR_HIP;
+            } else if (g) {
+                (void)hipGraphDestroy(g);
+            }
+            (void)hipStreamDestroy(cap);
+        }
+    }
+    if (rc != PERKS_OK) {
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        (void)p->dw.release();
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return PERKS_OK;
+}
+
+}  // namespace
+}  // namespace perks
+
+extern "C" {
+
+int perks_jacobi2d_batch_plan(int dtype, const perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts,
+                              int autotune, void* stream, perks_batch_plan** plan)
+{
+    if (!grids || ngrids <= 0 || !opts || !plan) return PERKS_ERR_INVALID_ARG;
+    if (dtype < PERKS_F64 || dtype > PERKS_BF16) return PERKS_ERR_INVALID_ARG;
+    *plan = nullptr;
+    return perks::plan_batch(dtype, grids, ngrids, opts, autotune, (hipStream_t)stream, plan);
+}
+
+int perks_jacobi2d_batch_run(perks_batch_plan* p, void* stream_v, perks_run_report* report)
+{
+    if (!p) return PERKS_ERR_INVALID_ARG;
+    hipStream_t stream = (hipStream_t)stream_v;
+    int dev = 0;
+    PERKS_HIP_TRY(hipGetDevice(&dev));
+    if (dev != p->device) return PERKS_ERR_INVALID_ARG;
+    hipEvent_t ev0, ev1;
+    PERKS_HIP_TRY(hipEventCreate(&ev0));
+    PERKS_HIP_TRY(hipEventCreate(&ev1));
+    PERKS_HIP_TRY(hipEventRecord(ev0, stream));
+    if (p->exec) PERKS_HIP_TRY(hipGraphLaunch(p->exec, stream));
+    PERKS_HIP_TRY(hipEventRecord(ev1, stream));
+    PERKS_HIP_TRY(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (p->opts.method != PERKS_METHOD_PLAIN) {
+        const int rc = perks::grid_sync_check(p->gs, stream);
+        if (rc != PERKS_OK) return rc;
+    }
+    perks::fill_report(report, ms, p->work, p->launch, p->cfg, p->dtype);
+    return PERKS_OK;
+}
+
+int perks_jacobi2d_batch_plan_info(const perks_batch_plan* p, int32_t* result_in_b, int ngrids, int32_t* config)
+{
+    if (!p) return PERKS_ERR_INVALID_ARG;
+    if (result_in_b) {
+        if (ngrids != (int)p->work.result_in_b.size()) return PERKS_ERR_INVALID_ARG;
+        for (int i = 0; i < ngrids; ++i) result_in_b[i] = p->work.result_in_b[i];
+    }
+    if (config) {
+        config[0] = p->cfg.tile;
+        config[1] = p->cfg.levels;
+        config[2] = p->launch.occ;
+        config[3] = p->launch.grid;
+        config[4] = p->tuned;
+        config[5] = p->work.npass;
+    }
+    return PERKS_OK;
+}
+
+void perks_jacobi2d_batch_plan_destroy(perks_batch_plan* p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)p->dw.release();
+    delete p;
+}
+
+}  // extern "C"

@@ -2,5 +2,8 @@
 #include "j2d_kernel.h"
 
 namespace perks {
-void* j2d_kernel_f16(int shape, int bc, bool has_div) { return j2dk::kptr<j2dk::f16s>(shape, bc, has_div); }
+void* j2d_kernel_f16(int shape, int bc, bool has_div, int tile)
+{
+    return j2dk::kptr<j2dk::f16s>(shape, bc, has_div, tile);
+}
 }  // namespace perks

@@ -196,42 +196,53 @@ __global__ void __launch_bounds__(256) j2d_kernel(J2dArgs a, KCoefs cf)
     }
 }
 
-template <typename S>
+// Tile shapes (TX x TY, two LDS tiles of them): variant 0 (128 x 32; 64 x 32
+// for FP64: four workgroups' LDS per CU, which hides latency best in
+// measurements at 8192^2) and variant 1, twice the height (less ghost-rim
+// work per cell, two workgroups per CU).
+template <typename S, int V>
 struct TileOf {
-    static constexpr int TX = 128, TY = 64;
+    static constexpr int TX = 128, TY = V == 0 ? 32 : 64;
 };
-template <>
-struct TileOf<double> {
-    static constexpr int TX = 64, TY = 64;
+template <int V>
+struct TileOf<double, V> {
+    static constexpr int TX = 64, TY = V == 0 ? 32 : 64;
 };
 
-template <int SHAPE, typename S, int BC, bool HD>
+template <int SHAPE, typename S, int BC, bool HD, int V>
 void* kptr3()
 {
-    return reinterpret_cast<void*>(&j2d_kernel<SHAPE, S, BC, TileOf<S>::TX, TileOf<S>::TY, HD>);
+    return reinterpret_cast<void*>(&j2d_kernel<SHAPE, S, BC, TileOf<S, V>::TX, TileOf<S, V>::TY, HD>);
 }
 
-template <int SHAPE, typename S>
+template <int SHAPE, typename S, int V>
 void* kptr2(int bc, bool hd)
 {
-    if (bc == PERKS_BC_ZERO) return hd ? kptr3<SHAPE, S, PERKS_BC_ZERO, true>() : kptr3<SHAPE, S, PERKS_BC_ZERO, false>();
-    return hd ? kptr3<SHAPE, S, PERKS_BC_CLAMP, true>() : kptr3<SHAPE, S, PERKS_BC_CLAMP, false>();
+    if (bc == PERKS_BC_ZERO)
+        return hd ? kptr3<SHAPE, S, PERKS_BC_ZERO, true, V>() : kptr3<SHAPE, S, PERKS_BC_ZERO, false, V>();
+    return hd ? kptr3<SHAPE, S, PERKS_BC_CLAMP, true, V>() : kptr3<SHAPE, S, PERKS_BC_CLAMP, false, V>();
+}
+
+template <typename S, int V>
+void* kptr_v(int shape, int bc, bool hd)
+{
+    switch (shape) {
+    case PERKS_SHAPE_2D_STAR_R1: return kptr2<PERKS_SHAPE_2D_STAR_R1, S, V>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R2: return kptr2<PERKS_SHAPE_2D_STAR_R2, S, V>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R3: return kptr2<PERKS_SHAPE_2D_STAR_R3, S, V>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R4: return kptr2<PERKS_SHAPE_2D_STAR_R4, S, V>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R5: return kptr2<PERKS_SHAPE_2D_STAR_R5, S, V>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R6: return kptr2<PERKS_SHAPE_2D_STAR_R6, S, V>(bc, hd);
+    case PERKS_SHAPE_2D_BOX_R1: return kptr2<PERKS_SHAPE_2D_BOX_R1, S, V>(bc, hd);
+    case PERKS_SHAPE_2D_BOX_R2: return kptr2<PERKS_SHAPE_2D_BOX_R2, S, V>(bc, hd);
+    default: return nullptr;
+    }
 }
 
 template <typename S>
-void* kptr(int shape, int bc, bool hd)
+void* kptr(int shape, int bc, bool hd, int tile)
 {
-    switch (shape) {
-    case PERKS_SHAPE_2D_STAR_R1: return kptr2<PERKS_SHAPE_2D_STAR_R1, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R2: return kptr2<PERKS_SHAPE_2D_STAR_R2, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R3: return kptr2<PERKS_SHAPE_2D_STAR_R3, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R4: return kptr2<PERKS_SHAPE_2D_STAR_R4, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R5: return kptr2<PERKS_SHAPE_2D_STAR_R5, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R6: return kptr2<PERKS_SHAPE_2D_STAR_R6, S>(bc, hd);
-    case PERKS_SHAPE_2D_BOX_R1: return kptr2<PERKS_SHAPE_2D_BOX_R1, S>(bc, hd);
-    case PERKS_SHAPE_2D_BOX_R2: return kptr2<PERKS_SHAPE_2D_BOX_R2, S>(bc, hd);
-    default: return nullptr;
-    }
+    return tile == 1 ? kptr_v<S, 1>(shape, bc, hd) : kptr_v<S, 0>(shape, bc, hd);
 }
 
 }  // namespace j2dk

@@ -246,6 +246,46 @@ def jacobi2d_device(d_a, d_b, width_y, width_x, steps, dtype="f64", stream=None,
     return rep
 
 
+class BatchPlan:
+    """A captured, optionally autotuned launch of a fixed batch of 2D grids
+    (perks_jacobi2d_batch_plan).  run() replays it; close() frees it."""
+
+    def __init__(self, grids, dtype="f32", autotune=True, stream=None, opts=None, **kw):
+        o = opts if opts is not None else make_opts(**kw)
+        arr = (L.Grid2d * len(grids))()
+        for i, (a, b, wy, wx, st) in enumerate(grids):
+            arr[i].a, arr[i].b = _ptr(a), _ptr(b)
+            arr[i].width_y, arr[i].width_x, arr[i].steps = int(wy), int(wx), int(st)
+        self._lib = L.load()
+        self._h = C.c_void_p()
+        self.n = len(grids)
+        s = None if stream is None else _ptr_stream(stream)
+        L.check(self._lib.perks_jacobi2d_batch_plan(_dt(dtype), arr, len(grids), C.byref(o), int(bool(autotune)), s,
+                                                    C.byref(self._h)), "jacobi2d_batch_plan")
+        flags = (C.c_int32 * self.n)()
+        cfg = (C.c_int32 * 6)()
+        L.check(self._lib.perks_jacobi2d_batch_plan_info(self._h, flags, self.n, cfg), "batch_plan_info")
+        self.result_in_b = list(flags)
+        self.config = dict(zip(("tile", "steps_per_pass", "wg_per_cu", "workgroups", "autotuned", "passes"), cfg))
+
+    def run(self, stream=None) -> L.RunReport:
+        rep = L.RunReport()
+        s = None if stream is None else _ptr_stream(stream)
+        L.check(self._lib.perks_jacobi2d_batch_run(self._h, s, C.byref(rep)), "jacobi2d_batch_run")
+        return rep
+
+    def close(self):
+        if self._h:
+            self._lib.perks_jacobi2d_batch_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def jacobi2d_batch(grids, dtype="f32", stream=None, opts=None, **kw):
     """grids: list of (d_a, d_b, width_y, width_x, steps).  Returns (report,
     list of result_in_b flags)."""

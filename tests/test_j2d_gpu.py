@@ -156,3 +156,41 @@ def test_2d_stochastic_rounding_matches_oracle(gpu, fmt, shape, method):
     got = (b if rep.result_in_b else a).cpu().numpy().view(np.uint16).reshape(dims)
     want = O.iterate16_sr(init, O.stencil_from_shape(si, "clamp", coefs=coefs), steps, fmt, 77)
     check_equal(got, want, f"{fmt} {shape} {method} stochastic")
+
+
+def _batch(torch, dtype, nb, seed):
+    rng = np.random.default_rng(seed)
+    specs = [(int(rng.integers(20, 180)), int(rng.integers(20, 260)), int(rng.integers(1, 30))) for _ in range(nb)]
+    np_dt = np.float64 if dtype == "f64" else np.float32
+    fields = [O.uniform(wy * wx, 300 + i, 0.0, 1.0, np_dt).reshape(wy, wx) for i, (wy, wx, _) in enumerate(specs)]
+    bufs = [(torch.from_numpy(f.copy()).view(-1).cuda(),
+             torch.empty(f.size, dtype=torch.from_numpy(f).dtype, device="cuda")) for f in fields]
+    return specs, fields, bufs
+
+
+@pytest.mark.parametrize("method,autotune", [("persistent", False), ("persistent", True), ("plain", False)])
+def test_2d_batch_plan_graph_replays(gpu, method, autotune):
+    """The captured launch replays several times with the program's own eager
+    work (resetting the inputs) between replays; every replay matches the oracle."""
+    torch = gpu
+    dtype = "f32"
+    specs, fields, bufs = _batch(torch, dtype, 10, 21)
+    say(f"[j2d] batch plan {method} autotune={autotune} {specs}")
+    si = P.shape_info("2d9pt")
+    coefs = [0.1, 0.1, 0.1, 0.1, 0.2, 0.1, 0.1, 0.1, 0.1]
+    plan = P.BatchPlan([(a, b, wy, wx, st) for (a, b), (wy, wx, st) in zip(bufs, specs)], dtype=dtype,
+                       autotune=autotune, shape="2d9pt", method=method, boundary="zero", coefs=coefs)
+    say(f"[j2d] plan config {plan.config}")
+    assert plan.config["autotuned"] == int(autotune and method == "persistent")
+    wants = [O.iterate(f, O.stencil_from_shape(si, "zero", coefs=coefs), st) for f, (_, _, st) in zip(fields, specs)]
+    for replay in range(3):
+        for (a, b), f in zip(bufs, fields):
+            a.copy_(torch.from_numpy(f.copy()).view(-1).cuda())
+            b.fill_(-1.0)
+        torch.cuda.synchronize()
+        rep = plan.run()
+        assert rep.elapsed_ms > 0
+        for i, ((a, b), (wy, wx, st)) in enumerate(zip(bufs, specs)):
+            got = (b if plan.result_in_b[i] else a).cpu().numpy().reshape(wy, wx)
+            check_equal(got, wants[i], f"replay {replay} grid {i} {wy}x{wx} steps {st}")
+    plan.close()
