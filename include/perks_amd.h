@@ -281,7 +281,8 @@ typedef struct perks_cg_opts {
     int32_t max_outer;   /* F32_REFINE: refinement steps (0 = 20)         */
     int32_t x0_is_zero;  /* 1: skip the initial A x (x must hold zeros)    */
     int32_t blkpsm;      /* cap on workgroups per CU (0 = occupancy)       */
-    int32_t reserved[5];
+    int32_t static_iter; /* 1: run exactly max_iter iterations (--staticiter, cg_kernels.cuh:834) */
+    int32_t reserved[4];
 } perks_cg_opts;
 
 typedef struct perks_cg_report {
@@ -306,6 +307,83 @@ int perks_cg_solve_csr(const perks_csr* A, const double* d_b, double* d_x, const
                        perks_cg_report* report);
 int perks_cg_solve_dia(const perks_dia* A, const double* d_b, double* d_x, const perks_cg_opts* opts, void* stream,
                        perks_cg_report* report);
+
+/* FP32 CG (the reference's myTest<float,...>): FP32 matrix and vectors, dot
+ * products accumulated in double as the reference's CgParams<float, double>
+ * does; the stopping threshold is (float)tol * (float)tol (times b.b when
+ * relative).  opts->precision is ignored. */
+typedef struct perks_csr_f32 {
+    int32_t n;
+    int32_t reserved0;
+    int64_t nnz;
+    const int32_t* rowptr;
+    const int32_t* col;
+    const float* val;
+} perks_csr_f32;
+typedef struct perks_dia_f32 {
+    int32_t n;
+    int32_t ndiag;
+    const int32_t* offsets; /* host array */
+    const float* val;
+} perks_dia_f32;
+int perks_cg_solve_csr_f32(const perks_csr_f32* A, const float* d_b, float* d_x, const perks_cg_opts* opts,
+                           void* stream, perks_cg_report* report);
+int perks_cg_solve_dia_f32(const perks_dia_f32* A, const float* d_b, float* d_x, const perks_cg_opts* opts,
+                           void* stream, perks_cg_report* report);
+
+/* ---- the reference's CG driver -------------------------------------------------
+ * myTest<ValueT,int,...> (conjugateGradient/cg_driver.cu:53-428): builds the
+ * matrix (a Matrix Market file, or the reference's virtual random tridiagonal
+ * matrix of vdata_n rows), sets rhs = 1 and x = 0, runs the persistent CG in
+ * FP64 (value_bytes 8) or FP32 (4) to ||r|| < tol (absolute, as the
+ * reference), optionally checks max|A x - rhs| on the host, and prints the
+ * reference's report lines when verbose.  Device memory is allocated and
+ * freed inside the call. */
+typedef struct perks_cg_driver_opts {
+    const char* mtx_path;  /* NULL or "": the virtual tridiagonal matrix (--vdata) */
+    int32_t value_bytes;   /* 8: double, 4: float                              */
+    int32_t vdata_n;       /* rows of the virtual matrix (0: 1048576)          */
+    int32_t max_iter;      /* --iters; <= 0: 10 * rows (file) or rows (virtual) */
+    int32_t static_iter;   /* --staticiter                                     */
+    int32_t warmup;        /* --warmup: one untimed 2-iteration solve first    */
+    int32_t check;         /* --check: max |A x - rhs| on the host             */
+    int32_t verbose;       /* print the reference's report lines              */
+    int32_t reserved0;
+    double tol;            /* 0: 1e-10                                         */
+} perks_cg_driver_opts;
+typedef struct perks_cg_driver_result {
+    int32_t n;
+    int32_t iterations;
+    int64_t nnz;
+    double residual;       /* sqrt(r.r) at exit                                */
+    double elapsed_ms;     /* device time of the timed solve                   */
+    double ms_per_iter;
+    double gbps;           /* the reference's byte model per iteration / time  */
+    double check_error;    /* max |A x - rhs| when check, else -1              */
+    int32_t grid;
+    int32_t code;          /* PERKS_OK or PERKS_ERR_NOT_CONVERGED              */
+} perks_cg_driver_result;
+int perks_cg_driver(const perks_cg_driver_opts* opts, perks_cg_driver_result* result);
+
+/* ---- sparse inputs (host) ----------------------------------------------------
+ * Matrix Market reader and COO -> CSR as the reference's CooMatrix::InitMarket /
+ * CsrMatrix::Init (conjugateGradient/matrix/sparse_matrix.h:217-390, 666-734):
+ * entries stable-sorted by (row, column), duplicates kept, symmetric files
+ * mirrored (skew: negated), pattern entries 1, array files column-major.
+ * The arrays are allocated by the library; free each with perks_free_host. */
+int perks_mm_read_csr(const char* path, int32_t* n, int64_t* nnz, int32_t** rowptr, int32_t** col, double** val);
+void perks_free_host(void* p);
+/* The reference's random symmetric tridiagonal matrix (conjugateGradient/
+ * matrix/tridiag.cpp:11-47, glibc rand()): n rows, nnz = 3n - 2, arrays of
+ * n+1 / nnz / nnz entries supplied by the caller. */
+int perks_gen_tridiag_f64(int32_t n, int32_t* rowptr, int32_t* col, double* val);
+int perks_gen_tridiag_f32(int32_t n, int32_t* rowptr, int32_t* col, float* val);
+/* CSR -> DIA on the host: offsets (ascending) of the diagonals that hold
+ * entries; dia_val (may be NULL) receives ndiag*n values (duplicate entries of
+ * one position added).  Returns ndiag, or PERKS_ERR_UNSUPPORTED when more
+ * than max_diag diagonals are needed. */
+int perks_csr_to_dia(int32_t n, const int32_t* rowptr, const int32_t* col, const double* val, int32_t max_diag,
+                     int32_t* offsets, double* dia_val);
 
 /* 3D Poisson 7-point operator on nx*ny*nz points (x fastest, Dirichlet):
  * diagonal 6, off-diagonals -1.  CSR arrays and the 7 DIA diagonals (offsets

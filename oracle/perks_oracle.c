@@ -583,3 +583,39 @@ int po_cg_f32_refine(int fmt, int n, const int32_t* rowptr, const int32_t* col, 
     free(valf); free(r); free(Ax); free(rf); free(bf); free(d); free(p); free(Ap);
     return total;
 }
+
+/* ---- FP32 CG (see perks_oracle.h) ------------------------------------------ */
+int po_cg_f32(int fmt, int n, const int32_t* rowptr, const int32_t* col, const float* val, int ndiag,
+              const float* b, float* x, double thr2, int relative, int max_iter, int static_iter,
+              double* hist, int hist_cap)
+{
+    float* r = (float*)malloc((size_t)n * sizeof(float));
+    float* p = (float*)malloc((size_t)n * sizeof(float));
+    float* Ap = (float*)malloc((size_t)n * sizeof(float));
+    spmv_f32(fmt, n, rowptr, col, val, ndiag, x, Ap);
+    for (int i = 0; i < n; ++i) r[i] = b[i] + (-1.0f) * Ap[i];
+    const double thr = relative ? thr2 * po_dot_f32(b, b, (size_t)n) : thr2;
+    double r1 = po_dot_f32(r, r, (size_t)n), r0 = 0.0;
+    if (hist && hist_cap > 0) hist[0] = r1;
+    int k = 1;
+    while ((static_iter || !(r1 < thr)) && k <= max_iter) {
+        if (k > 1) {
+            const float beta = (float)(r1 / r0);
+            for (int i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
+        } else {
+            memcpy(p, r, (size_t)n * sizeof(float));
+        }
+        spmv_f32(fmt, n, rowptr, col, val, ndiag, p, Ap);
+        const double pAp = po_dot_f32(p, Ap, (size_t)n);
+        const float alpha = (float)(r1 / pAp);
+        const float na = -alpha;
+        for (int i = 0; i < n; ++i) x[i] = alpha * p[i] + x[i];
+        for (int i = 0; i < n; ++i) r[i] = na * Ap[i] + r[i];
+        r0 = r1;
+        r1 = po_dot_f32(r, r, (size_t)n);
+        if (hist && k < hist_cap) hist[k] = r1;
+        ++k;
+    }
+    free(r); free(p); free(Ap);
+    return k - 1;
+}

@@ -85,6 +85,14 @@ float po_bf16_to_f32(uint16_t b);
 void po_stencil_iterate_f16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps);
 void po_stencil_iterate_bf16s(const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const po_stencil* st, int steps);
 
+/* FP32 CG (float matrix and vectors, dot products in double, same order as
+ * po_cg_f64), as the reference's myTest<float,...>.  thr2: the stopping
+ * threshold on r.r (already squared; times b.b when relative).  static_iter:
+ * run exactly max_iter iterations.  Returns the iterations run. */
+int po_cg_f32(int fmt, int n, const int32_t* rowptr, const int32_t* col, const float* val, int ndiag,
+              const float* b, float* x, double thr2, int relative, int max_iter, int static_iter,
+              double* hist, int hist_cap);
+
 /* Stochastic rounding of 16-bit storage (this library's addition; no
  * reference counterpart).  The random bits of cell `idx` (linear index
  * z*ny*nx + y*nx + x, taken mod 2^32) at time step t (0-based) are

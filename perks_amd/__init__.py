@@ -5,13 +5,14 @@ include/perks_amd.h).  This package is the host-side mirror of the reference's
 entry points; it loads the library and fails loudly if it is missing.
 """
 from . import _lib
-from .cg import CgResult, cg_solve, poisson7_csr, poisson7_dia, poisson7_nnz
+from .cg import (CgResult, cg_driver, cg_solve, csr_to_dia, gen_tridiag, mm_read_csr, poisson7_csr, poisson7_dia,
+                 poisson7_nnz)
 from ._lib import PerksError, check, load
 from .stencil import (BatchPlan, fill_uniform, j3d_device, j3d_iterative, j3d_run, jacobi2d_batch, jacobi2d_device,
                       jacobi2d_run, jacobi_iterative, make_opts, plan, query_device, shape_id, shape_info)
 
 __all__ = [
-    "BatchPlan", "CgResult", "cg_solve", "poisson7_csr", "poisson7_dia", "poisson7_nnz",
+    "BatchPlan", "CgResult", "cg_driver", "cg_solve", "csr_to_dia", "gen_tridiag", "mm_read_csr", "poisson7_csr", "poisson7_dia", "poisson7_nnz",
     "PerksError", "check", "load", "fill_uniform", "j3d_device", "j3d_iterative", "j3d_run", "jacobi2d_batch",
     "jacobi2d_device", "jacobi2d_run", "jacobi_iterative", "make_opts", "plan", "query_device", "shape_id",
     "shape_info", "_lib",

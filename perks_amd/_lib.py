@@ -101,7 +101,20 @@ CG_F64, CG_F32_REFINE = 0, 1
 class CgOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("relative", C.c_int32), ("max_iter", C.c_int32),
                 ("precision", C.c_int32), ("max_inner", C.c_int32), ("max_outer", C.c_int32),
-                ("x0_is_zero", C.c_int32), ("blkpsm", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("x0_is_zero", C.c_int32), ("blkpsm", C.c_int32), ("static_iter", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
+class CgDriverOpts(C.Structure):
+    _fields_ = [("mtx_path", C.c_char_p), ("value_bytes", C.c_int32), ("vdata_n", C.c_int32),
+                ("max_iter", C.c_int32), ("static_iter", C.c_int32), ("warmup", C.c_int32), ("check", C.c_int32),
+                ("verbose", C.c_int32), ("reserved0", C.c_int32), ("tol", C.c_double)]
+
+
+class CgDriverResult(C.Structure):
+    _fields_ = [("n", C.c_int32), ("iterations", C.c_int32), ("nnz", C.c_int64), ("residual", C.c_double),
+                ("elapsed_ms", C.c_double), ("ms_per_iter", C.c_double), ("gbps", C.c_double),
+                ("check_error", C.c_double), ("grid", C.c_int32), ("code", C.c_int32)]
 
 
 class CgReport(C.Structure):
@@ -147,6 +160,15 @@ SIGNATURES = {
     "perks_cg_opts_init": (None, [C.POINTER(CgOpts)]),
     "perks_cg_solve_csr": (_I, [C.POINTER(Csr), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),
     "perks_cg_solve_dia": (_I, [C.POINTER(Dia), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),
+    "perks_cg_solve_csr_f32": (_I, [C.POINTER(Csr), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),
+    "perks_cg_solve_dia_f32": (_I, [C.POINTER(Dia), _P, _P, C.POINTER(CgOpts), _P, C.POINTER(CgReport)]),
+    "perks_cg_driver": (_I, [C.POINTER(CgDriverOpts), C.POINTER(CgDriverResult)]),
+    "perks_mm_read_csr": (_I, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_void_p),
+                               C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "perks_free_host": (None, [C.c_void_p]),
+    "perks_gen_tridiag_f64": (_I, [C.c_int32, _P, _P, _P]),
+    "perks_gen_tridiag_f32": (_I, [C.c_int32, _P, _P, _P]),
+    "perks_csr_to_dia": (_I, [C.c_int32, _P, _P, _P, C.c_int32, _P, _P]),
     "perks_poisson7_nnz": (C.c_int64, [_I, _I, _I]),
     "perks_poisson7_csr": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "perks_poisson7_dia": (_I, [_I, _I, _I, _P, _P, _P]),

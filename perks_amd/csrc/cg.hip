@@ -81,6 +81,7 @@ struct CgArgs {
     GridSync* sync;
     unsigned long long timeout;
     int x_is_zero;      // skip the initial SpMV
+    int static_iter;    // run exactly max_iter iterations
 };
 
 // Folds the 256 per-lane values of a workgroup pairwise with strides
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(kLanes) cg_persistent_kernel(CgArgs<V> a)
     int k = 1;
     V* pold = a.p0;
     V* pnew = a.p1;
-    while (!(r1 < thr) && k <= a.max_iter) {
+    while ((a.static_iter || !(r1 < thr)) && k <= a.max_iter) {
         // ---- p = r + beta*p ; Ap = A p ; partials of p.Ap ----
         if (k == 1) {
             phase_spmv<V, FMT, true>(a, pold, pnew, (V)0, lds);
@@ -445,6 +446,7 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
         a.hist = S.hist;
         a.hist_cap = hist_cap;
         a.x_is_zero = o->x0_is_zero;
+        a.static_iter = o->static_iter;
         rc = launch_cg_fmt<double>(fmt, a, stream, o->blkpsm, &grid);
         if (rc != PERKS_OK) return rc;
     } else {
@@ -583,7 +585,93 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
     if (rep->history && ncopy > 0) std::memcpy(rep->history, hist_host.data(), sizeof(double) * (size_t)ncopy);
     rep->hist_len = (int)hist_host.size();
     const double thr = o->relative ? o->tol * o->tol * bb : o->tol * o->tol;
-    return rr < thr ? PERKS_OK : PERKS_ERR_NOT_CONVERGED;
+    return (rr < thr || (!refine && o->static_iter)) ? PERKS_OK : PERKS_ERR_NOT_CONVERGED;
+}
+
+// FP32 CG on FP32 data (the reference's myTest<float,...>).
+int cg_solve_f32(int fmt, const perks_csr_f32* csr, const perks_dia_f32* dia, const float* d_b, float* d_x,
+                 const perks_cg_opts* o, void* stream_v, perks_cg_report* rep)
+{
+    hipStream_t stream = (hipStream_t)stream_v;
+    if (!o || !d_b || !d_x || !rep) return PERKS_ERR_INVALID_ARG;
+    const int n = fmt == FMT_CSR ? (csr ? csr->n : -1) : (dia ? dia->n : -1);
+    if (n <= 0 || o->max_iter < 0) return PERKS_ERR_INVALID_ARG;
+    if (fmt == FMT_DIA && (dia->ndiag <= 0 || dia->ndiag > kMaxDiag)) return PERKS_ERR_UNSUPPORTED;
+    int dev = 0;
+    PERKS_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return PERKS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g_cg_mu);
+    CgScratch& S = g_cg[dev];
+    const int hist_cap = std::max(1, o->max_iter + 1);
+    int rc = ensure(S, (size_t)n, 0, hist_cap);
+    if (rc != PERKS_OK) return rc;
+    CgArgs<float> f;
+    std::memset(&f, 0, sizeof(f));
+    f.n = n;
+    f.nseg = S.nseg;
+    if (fmt == FMT_CSR) {
+        f.rowptr = csr->rowptr;
+        f.col = csr->col;
+        f.val = csr->val;
+    } else {
+        f.ndiag = dia->ndiag;
+        for (int d = 0; d < dia->ndiag; ++d) f.offs[d] = dia->offsets[d];
+        f.val = dia->val;
+    }
+    f.x = d_x;
+    f.r = (float*)S.r;
+    f.p0 = (float*)S.p0;
+    f.p1 = (float*)S.p1;
+    f.ap = (float*)S.ap;
+    f.b = d_b;
+    const float tf = (float)o->tol;
+    f.tol2 = (double)(tf * tf);
+    f.relative = o->relative;
+    f.max_iter = o->max_iter;
+    f.hist = S.hist;
+    f.hist_cap = hist_cap;
+    f.part_a = S.parts;
+    f.part_b = S.parts + S.nseg;
+    f.rr_out = S.small;
+    f.bb_out = S.small + 1;
+    f.iters_out = (int*)(S.small + 2);
+    f.x_is_zero = o->x0_is_zero;
+    f.static_iter = o->static_iter;
+    hipEvent_t ev0, ev1;
+    PERKS_HIP_TRY(hipEventCreate(&ev0));
+    PERKS_HIP_TRY(hipEventCreate(&ev1));
+    PERKS_HIP_TRY(hipEventRecord(ev0, stream));
+    int grid = 0;
+    rc = launch_cg_fmt<float>(fmt, f, stream, o->blkpsm, &grid);
+    if (rc != PERKS_OK) return rc;
+    PERKS_HIP_TRY(hipEventRecord(ev1, stream));
+    PERKS_HIP_TRY(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    GridSync* gs = nullptr;
+    rc = grid_sync_get(kSyncCg, &gs);
+    if (rc != PERKS_OK) return rc;
+    rc = grid_sync_check(gs, stream);
+    if (rc != PERKS_OK) return rc;
+    double small[3];
+    PERKS_HIP_TRY(hipMemcpy(small, S.small, sizeof(small), hipMemcpyDeviceToHost));
+    int it = 0;
+    std::memcpy(&it, &small[2], sizeof(int));
+    std::vector<double> hist((size_t)it + 1);
+    PERKS_HIP_TRY(hipMemcpy(hist.data(), S.hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost));
+    rep->iterations = it;
+    rep->outer_iterations = 0;
+    rep->final_rr = small[0];
+    rep->bb = small[1];
+    rep->elapsed_ms = ms;
+    rep->grid = grid;
+    const int ncopy = std::min<int>((int)hist.size(), rep->hist_cap);
+    if (rep->history && ncopy > 0) std::memcpy(rep->history, hist.data(), sizeof(double) * (size_t)ncopy);
+    rep->hist_len = (int)hist.size();
+    const double thr = o->relative ? f.tol2 * small[1] : f.tol2;
+    return (small[0] < thr || o->static_iter) ? PERKS_OK : PERKS_ERR_NOT_CONVERGED;
 }
 
 int release_cg_scratch()
@@ -710,6 +798,20 @@ int perks_cg_solve_dia(const perks_dia* A, const double* d_b, double* d_x, const
 {
     if (!A || !A->offsets || !A->val) return PERKS_ERR_INVALID_ARG;
     return perks::cg_solve(1, nullptr, A, d_b, d_x, opts, stream, report);
+}
+
+int perks_cg_solve_csr_f32(const perks_csr_f32* A, const float* d_b, float* d_x, const perks_cg_opts* opts,
+                           void* stream, perks_cg_report* report)
+{
+    if (!A || !A->rowptr || !A->col || !A->val) return PERKS_ERR_INVALID_ARG;
+    return perks::cg_solve_f32(0, A, nullptr, d_b, d_x, opts, stream, report);
+}
+
+int perks_cg_solve_dia_f32(const perks_dia_f32* A, const float* d_b, float* d_x, const perks_cg_opts* opts,
+                           void* stream, perks_cg_report* report)
+{
+    if (!A || !A->offsets || !A->val) return PERKS_ERR_INVALID_ARG;
+    return perks::cg_solve_f32(1, nullptr, A, d_b, d_x, opts, stream, report);
 }
 
 int64_t perks_poisson7_nnz(int nx, int ny, int nz)

@@ -46,6 +46,24 @@ def main():
         f[0], f[-1], f[:, 0], f[:, -1], f[:, :, 0], f[:, :, -1] = 0, 0, 0, 0, 0, 0
         out[f"j3d_gold4_{tag}_in"] = f
         out[f"j3d_gold4_{tag}_out"] = O.ref_j3d_gold_4step(f)
+    # genTridiag (conjugateGradient/matrix/tridiag.cpp:11-47), float and double,
+    # from the C library's rand() seeded as in a fresh process (srand(1)).
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    tri = ctypes.CDLL(str(O.REF_DIR / "libref_tridiag.so"))
+    n_tri = 50
+    nz = (n_tri - 2) * 3 + 4
+    for tag, dt, sym in (("f32", np.float32, "_Z10genTridiagIfiEvPT0_S1_PT_S0_S0_"),
+                         ("f64", np.float64, "_Z10genTridiagIdiEvPT0_S1_PT_S0_S0_")):
+        rowptr = np.zeros(n_tri + 1, np.int32)
+        col = np.zeros(nz, np.int32)
+        val = np.zeros(nz, dt)
+        libc.srand(1)
+        fn = getattr(tri, sym)
+        fn.restype = None
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        fn(rowptr.ctypes.data, col.ctypes.data, val.ctypes.data, n_tri, nz)
+        out[f"tridiag_{tag}_rowptr"], out[f"tridiag_{tag}_col"], out[f"tridiag_{tag}_val"] = rowptr, col, val
     np.savez(HERE / "ref_gold.npz", **out)
     print(f"wrote {HERE / 'ref_gold.npz'}: {len(out)} arrays")
 

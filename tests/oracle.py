@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
             "po_stencil_iterate_f16s": (None, [P, P, I, I, I, P, I]),
             "po_stencil_iterate_bf16s": (None, [P, P, I, I, I, P, I]),
             "po_lowbias32": (C.c_uint32, [C.c_uint32]),
+            "po_cg_f32": (I, [I, I, P, P, P, I, P, P, C.c_double, I, I, I, P, I]),
             "po_sr_bits": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
             "po_f32_to_bf16_sr": (C.c_uint16, [C.c_float, C.c_uint32]),
             "po_f32_to_f16_sr": (C.c_uint16, [C.c_float, C.c_uint32]),
@@ -270,6 +271,15 @@ def ref_j3d_gold_4step(field: np.ndarray) -> np.ndarray:
     f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     f(p(field), *field.shape, p(out))
     return out
+
+
+def cg_f32(fmt, n, rowptr, col, val32, ndiag, b32, thr2, relative, max_iter, static_iter=False, x0=None):
+    """FP32 CG (float data, double dots); thr2 is the squared threshold."""
+    x = np.zeros(n, np.float32) if x0 is None else np.array(x0, dtype=np.float32)
+    hist = np.zeros(max_iter + 1)
+    k = lib().po_cg_f32(fmt, n, p(rowptr) if rowptr is not None else None, p(col), p(val32), ndiag, p(b32), p(x),
+                        thr2, int(relative), max_iter, int(static_iter), p(hist), hist.size)
+    return x, hist[: k + 1], k
 
 
 def cg_f32_refine(fmt, n, rowptr, col, val, ndiag, b, tol, relative, inner_tol, max_inner, max_outer):

@@ -133,3 +133,31 @@ int main(int argc, char**) {
                     str(L.LIB_PATH), f"-Wl,-rpath,{L.LIB_PATH.parent}"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     assert "gfx950" in out
+
+
+def test_cg_driver_dropin_header_compiles_and_links(tmp_path):
+    """The reference's conjugateGradient main calls myTest<ValueT,int,...>(devID,
+    deviceProp, args); the drop-in header provides it over the C ABI."""
+    src = tmp_path / "cgmain.cpp"
+    src.write_text(r'''
+#include "perks_amd/cg_driver.hpp"
+#include <cstdio>
+int main(int argc, char** argv) {
+    CommandLineArgs args(argc, argv);
+    perks_device_caps prop{};
+    if (args.CheckCmdLineFlag("never-given")) {  // link check only, no device call
+        myTest<double, int, true, false, false>(0, prop, args);
+        myTest<float, int, false, true, true>(0, prop, args);
+    }
+    int iters = -1;
+    args.GetCmdLineArgument("iters", iters);
+    std::printf("%d %d\n", iters, (int)args.CheckCmdLineFlag("check"));
+    return 0;
+}
+''')
+    exe = tmp_path / "cgmain"
+    subprocess.run(["g++", "-std=c++14", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
+                    str(L.LIB_PATH), f"-Wl,-rpath,{L.LIB_PATH.parent}"], check=True)
+    out = subprocess.run([str(exe), "--iters=37", "--check", "--mtx=x.mtx"], capture_output=True, text=True,
+                         check=True).stdout
+    assert out.split() == ["37", "1"]

@@ -150,3 +150,84 @@ def test_cg_f32_refinement_matches_oracle(gpu, fmt, dims):
     assert np.array_equal(res.history, want_hist)
     assert np.array_equal(x.cpu().numpy(), want_x)
     assert res.relative_residual < 1e-10
+
+
+@pytest.mark.parametrize("fmt", ["csr", "dia"])
+@pytest.mark.parametrize("relative", [True, False])
+def test_cg_f32_matches_oracle(gpu, fmt, relative):
+    """FP32 CG (the reference's myTest<float,...>): float data, double dots."""
+    torch = gpu
+    dims = (30, 26, 22)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    say(f"[cg] f32 {fmt} relative={relative}")
+    tol = 1e-5 if relative else 1e-3
+    b_host = O.uniform(n, 17, -1.0, 1.0, np.float32)
+    if fmt == "csr":
+        rowptr, col, val = O.poisson7_csr(*dims)
+        val32 = val.astype(np.float32)
+        m = dict(rowptr=torch.from_numpy(rowptr).cuda(), col=torch.from_numpy(col).cuda(),
+                 val=torch.from_numpy(val32).cuda(), nnz=len(col))
+        oargs = (0, rowptr, col, val32, 0)
+    else:
+        offs, dval = O.poisson7_dia(*dims)
+        val32 = dval.astype(np.float32)
+        m = dict(offsets=offs, val=torch.from_numpy(val32).cuda())
+        oargs = (1, None, offs, val32, 7)
+    x = torch.zeros(n, dtype=torch.float32, device="cuda")
+    res = P.cg_solve(fmt, n, torch.from_numpy(b_host).cuda(), x, tol=tol, relative=relative, max_iter=3000,
+                     precision="f32", **m)
+    f, rp, cl, vv, nd = oargs
+    thr2 = float(np.float32(tol) * np.float32(tol))
+    want_x, want_hist, want_k = O.cg_f32(f, n, rp, cl, vv, nd, b_host, thr2, relative, 3000)
+    assert res.code == 0 and res.iterations == want_k
+    assert np.array_equal(res.history, want_hist)
+    assert np.array_equal(x.cpu().numpy(), want_x)
+
+
+def test_cg_static_iterations(gpu):
+    torch = gpu
+    dims = (16, 16, 16)
+    n = 16 ** 3
+    say("[cg] static iterations")
+    b = torch.from_numpy(O.uniform(n, 3, -1.0, 1.0)).cuda()
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    res = P.cg_solve("dia", n, b, x, tol=1e-3, max_iter=12, static_iter=True, **device_poisson(torch, "dia", *dims))
+    assert res.code == 0 and res.iterations == 12 and len(res.history) == 13
+
+
+@pytest.mark.parametrize("value_bytes", [8, 4])
+def test_cg_driver_virtual_tridiagonal(gpu, value_bytes):
+    """The reference's driver on its virtual matrix: rhs = 1, x = 0, absolute tol 1e-10."""
+    say(f"[cg] driver vdata value_bytes={value_bytes}")
+    r = P.cg_driver(value_bytes=value_bytes, vdata_n=4096, check=True, max_iter=200)
+    assert r.n == 4096 and r.nnz == 3 * 4096 - 2
+    assert r.iterations > 0 and r.elapsed_ms > 0
+    if value_bytes == 8:
+        assert r.code == 0 and r.residual < 1e-10 and r.check_error < 1e-9
+    else:
+        assert r.check_error < 1e-4
+
+
+def test_cg_driver_matrix_market_file(gpu, tmp_path):
+    """A Matrix Market file of the Poisson operator through the driver: the same
+    iterations and residual as the CSR solve on the device-generated matrix."""
+    torch = gpu
+    dims = (12, 10, 9)
+    n = 12 * 10 * 9
+    rowptr, col, val = O.poisson7_csr(*dims)
+    f = tmp_path / "poisson.mtx"
+    rows = np.repeat(np.arange(n), np.diff(rowptr))
+    lower = col <= rows
+    lines = [f"{c + 1} {r + 1} {float(v)!r}" for r, c, v in zip(rows[lower], col[lower], val[lower])]
+    f.write_text("%%MatrixMarket matrix coordinate real symmetric\n" + f"{n} {n} {len(lines)}\n" + "\n".join(lines)
+                 + "\n")
+    say("[cg] driver mtx")
+    r = P.cg_driver(mtx=str(f), value_bytes=8, check=True)
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    res = P.cg_solve("csr", n, b, x, tol=1e-10, relative=False, max_iter=10 * n, x0_is_zero=True,
+                     **device_poisson(torch, "csr", *dims))
+    assert r.code == 0 and r.iterations == res.iterations
+    assert abs(r.residual - np.sqrt(res.final_rr)) == 0.0
+    assert r.check_error < 1e-9
