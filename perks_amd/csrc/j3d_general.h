@@ -12,6 +12,10 @@ namespace perks {
 // result is in d_b when rep->result_in_b.
 int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
             const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);
+// Host-only plan (no device call): grid, tile, LDS bytes; reserved[0] = steps
+// per pass, reserved[1] = z-slabs per tile column.
+int j3g_plan(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, const perks_device_caps* caps,
+             int occupancy, perks_plan_info* out);
 // Time steps per pass the persistent method uses for this shape and format.
 int j3g_levels(int dtype, int shape, const perks_stencil_opts* o);
 

@@ -81,6 +81,42 @@ int j3g_levels(int dtype, int shape, const perks_stencil_opts* o)
     return std::max(1, std::min(L, maxL));
 }
 
+int j3g_plan(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, const perks_device_caps* caps,
+             int occupancy, perks_plan_info* out)
+{
+    const ShapeDesc* sd = shape_by_id(o->shape);
+    if (!sd || sd->dims != 3 || !caps || !out) return PERKS_ERR_INVALID_ARG;
+    int tx, ty;
+    tile_of(dtype, &tx, &ty);
+    const int R = sd->radius;
+    const int L = j3g_levels(dtype, o->shape, o);
+    const int VW = tx - 2 * R * L, VH = ty - 2 * R * L;
+    const long long tiles_xy = (long long)((nx + VW - 1) / VW) * ((ny + VH - 1) / VH);
+    std::memset(out, 0, sizeof(*out));
+    out->block = j3g::kThreads;
+    out->tile_x = tx;
+    out->tile_y = ty;
+    out->lds_bytes = (int)lds_for(dtype, R, L);
+    out->reserved[0] = L;
+    if (o->method == PERKS_METHOD_PLAIN) {
+        const int nzs = (int)std::max(1LL, std::min<long long>(nz, (8LL * caps->cu_count + tiles_xy - 1) / tiles_xy));
+        out->grid[0] = (int)std::min<long long>(tiles_xy * nzs, 0x7fffffffLL);
+        out->reserved[1] = nzs;
+    } else {
+        if (occupancy <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
+        int occ = occupancy;
+        if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
+        if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
+        const int G = caps->cu_count * occ;
+        const int nzs = best_slabs(nz, tiles_xy, G, 2 * R * L);
+        out->occupancy = occ;
+        out->grid[0] = (int)std::min<long long>(G, tiles_xy * nzs);
+        out->reserved[1] = nzs;
+    }
+    out->grid[1] = out->grid[2] = 1;
+    return PERKS_OK;
+}
+
 int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
             const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep)
 {

@@ -32,6 +32,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -125,8 +126,39 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     return PERKS_OK;
 }
 
-int min_height_persistent(int, int, int, const perks_stencil_opts*) { return PERKS_ERR_UNSUPPORTED; }
-int min_width_y_persistent(int, int, int, int, int) { return PERKS_ERR_UNSUPPORTED; }
+// The reference's getminHeight (j3d.cu:598,612) is the domain height at which
+// its persistent grid's register and shared-memory caches hold every plane.
+// Here the kernels stream planes and keep only ring buffers on chip, so the
+// equivalent figure is the height whose field fits in the chip's aggregate
+// on-chip storage (LDS plus VGPRs of every CU): the size below which a
+// persistent kernel could keep the whole domain resident.
+int min_height_caps(int dtype, int ny, int nx, const perks_device_caps* caps)
+{
+    if (ny <= 0 || nx <= 0 || !caps) return PERKS_ERR_INVALID_ARG;
+    const long long onchip = (long long)caps->cu_count *
+                             ((long long)caps->lds_per_cu +
+                              (long long)caps->vgprs_per_simd * caps->simds_per_cu * caps->warp_size * 4LL);
+    const long long plane = (long long)ny * nx * elem_bytes(dtype);
+    const long long h = onchip / plane;
+    if (h < 3) return PERKS_ERR_HEIGHT_TOO_SMALL;
+    return (int)std::min<long long>(h, 0x7fffffffLL);
+}
+
+int min_height_persistent(int dtype, int ny, int nx, const perks_stencil_opts*)
+{
+    perks_device_caps caps;
+    const int rc = device_caps_cached(&caps);
+    if (rc != PERKS_OK) return rc;
+    return min_height_caps(dtype, ny, nx, &caps);
+}
+
+// getMinWidthY (j3d.cu:69-123) is the largest of getminHeight over the
+// workgroups-per-CU and cache variants; the on-chip capacity above does not
+// depend on them.
+int min_width_y_persistent(int dtype, int nx, int ny, int, int)
+{
+    return min_height_persistent(dtype, ny, nx, nullptr);
+}
 
 // ---- launch -----------------------------------------------------------------
 

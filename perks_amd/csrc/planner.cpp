@@ -6,6 +6,7 @@
 // that every refusal can be exercised on a machine without a GPU.
 #include <cstring>
 
+#include "j3d_general.h"
 #include "perks_amd.h"
 #include "perks_shapes.h"
 #include "planner.h"
@@ -54,6 +55,8 @@ extern "C" int perks_j3d_plan(int dtype, int height, int width_y, int width_x, c
         if (rc != PERKS_OK) return rc;
         caps = &local;
     }
+    if (opts->shape != PERKS_SHAPE_3D_STAR_R1 || opts->tune[5] == 1)
+        return perks::j3g_plan(dtype, height, width_y, width_x, opts, caps, occupancy_hint, out);
     if (opts->method == PERKS_METHOD_PLAIN) return perks::plan_plain(height, width_y, width_x, caps->cu_count, out);
     return perks::plan_persistent(dtype, height, width_y, width_x, opts, caps, occupancy_hint, out);
 }

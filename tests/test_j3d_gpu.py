@@ -155,3 +155,18 @@ def test_matches_committed_interior_gold(gpu, tag, method):
     say(f"[j3d] interior gold 3d7pt {tag} {method}")
     got, _ = P.j3d_run(np.ascontiguousarray(f[1:-1, 1:-1, 1:-1]), 4, method=method, boundary="zero")
     assert np.array_equal(got, GOLD[f"j3d_gold4_{tag}_out"][1:-1, 1:-1, 1:-1])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_min_height_is_the_on_chip_capacity(gpu, dtype):
+    """getminHeight / getMinWidthY: the height whose field fits the chip's LDS + VGPRs."""
+    caps = P.query_device()
+    onchip = caps.cu_count * (caps.lds_per_cu + caps.vgprs_per_simd * caps.simds_per_cu * caps.warp_size * 4)
+    wy, wx = 512, 512
+    plane = wy * wx * np.dtype(dtype).itemsize
+    h = P.j3d_iterative(np.empty(0, dtype), 0, wy, wx, None, getminHeight=True)
+    assert h * plane <= onchip < (h + 1) * plane
+    lib = P.load()
+    dt = 0 if dtype == np.float64 else 1
+    assert lib.perks_get_min_width_y(dt, wx, wy, 256, 0) == h
+    assert lib.perks_get_min_width_y(dt, 8192, 8192, 256, 0) == -4  # not even three planes fit
