@@ -551,6 +551,13 @@ def run_cg(args):
         torch.cuda.empty_cache()
     r = rows["csr_f64"]
     achieved = r["achieved_gbs"]
+    traffic = None
+    try:
+        t = json.loads((ROOT / "profiles" / "cg_csr_f64_256_traffic.json").read_text())
+        if t.get("n") == n and t.get("iterations") == r["iterations"]:
+            traffic = t.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        pass
     out = {
         "metric": "CG time-to-tolerance, 7pt Poisson 256^3, CSR/DIA, FP64 vs FP32-inner/FP64-refinement",
         "value": r["time_to_tolerance_ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
@@ -560,7 +567,7 @@ def run_cg(args):
                                f"uniform[-1,1) (seed {args.seed}), x0=0, ||r||/||b|| < 1e-10; value = CSR FP64",
                    "n": n, "nnz": nnz, "results": rows},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "cg_persistent_kernel<double, CSR>",
                      "algorithmic_bytes_per_launch": cg_bytes_per_iter("csr", n, nnz, 8) * r["iterations"]},
     }
