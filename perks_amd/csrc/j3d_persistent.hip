@@ -46,9 +46,9 @@ namespace {
 
 using j3dp::kMaxLevels;
 using j3dp::kNumVariants;
-using j3dp::kVariants;
 using j3dp::PersistArgs;
 using j3dp::Variant;
+using j3dp::variant_shape;
 
 void* pick_kernel(int fmt, int bc, bool has_div, int var, bool sr)
 {
@@ -115,7 +115,7 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     out->grid[0] = G;
     out->grid[1] = 1;
     out->grid[2] = 1;
-    const Variant v = kVariants[variant_of(o)];
+    const Variant v = variant_shape(dtype, variant_of(o));
     out->block = v.nw * 64;
     out->occupancy = occ;
     out->lds_bytes = (int)(2 * (kMaxLevels - 1) * v.nw * 2 * 64 * (dtype == PERKS_F64 ? 8 : 4));
@@ -178,7 +178,7 @@ int run_persistent_fmt(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, 
     if (!fn) return PERKS_ERR_UNSUPPORTED;
     int occ = 0;
     PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn),
-                                                               kVariants[var].nw * 64, 0));
+                                                               variant_shape(dtype, var).nw * 64, 0));
     perks_plan_info pl;
     rc = plan_persistent(dtype, nz, ny, nx, o, &caps, occ, &pl);
     if (rc != PERKS_OK) return rc;

@@ -90,12 +90,21 @@ struct Elem<PERKS_BF16> : Elem16<PERKS_BF16> {};
 // one halo row on each side at level 0; from level 1 on, the rows that cross
 // between waves come from the LDS exchange, and the workgroup's outermost
 // rows are ghosts, so a tile stores VH = NW*NY - 2(L-1) rows.
+//
+// Level l at iteration i computes plane i-l from level-(l-1) planes i-l-1,
+// i-l and i-l+1.  Of these only the centre plane i-l needs the rows outside
+// the wave's strip, and it was computed one iteration earlier: every level
+// publishes its two edge rows to LDS (buffer parity i&1) and the level above
+// reads the neighbours' rows of its centre plane from the other parity one
+// iteration later.  So the rings keep NY rows per plane, and one workgroup
+// barrier per iteration, after all levels, separates the writes of one
+// parity from its reads and the reads from the next overwrite.
 template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
 struct March {
     using E = Elem<FMT>;
     using S = typename E::S;
     using T = typename E::T;
-    static constexpr int R = NY + 2;  // rows held per plane: the strip and one halo row each side
+    static constexpr int R = NY + 2;  // level-0 rows held per plane: the strip and one halo row each side
     static constexpr int D = P - 3;   // level-0 planes loaded ahead of the one level 1 needs next
     static constexpr int UF = P % 3 == 0 ? P : 3 * P;  // stages per loop trip (ring slots are static)
     const S* __restrict__ src;
@@ -120,7 +129,7 @@ struct March {
     uint32_t key[L + 1];
 
     T w0[P][R];
-    T wl[L > 1 ? L - 1 : 1][3][R];
+    T wl[L > 1 ? L - 1 : 1][3][NY];
 
     __device__ __forceinline__ void load0(int q, int slot)
     {
@@ -130,20 +139,24 @@ struct March {
         for (int r = 0; r < R; ++r) w0[slot][r] = E::load(rs, roff[r]);
     }
 
-    // Level-l plane p from the three level-(l-1) planes b (p-1), c (p), u (p+1).
-    // EDGE adds the boundary rule at domain faces; interior waves skip it.
-    template <bool EDGE>
-    __device__ __forceinline__ void compute(const T (&b)[R], const T (&c)[R], const T (&u)[R], int p, int lv,
-                                            T (&out)[NY]) const
+    // Level-lv plane p from the level-(lv-1) planes b (p-1), c (p) and u
+    // (p+1), given by the wave's NY rows each (at offset `o` in their arrays)
+    // and the two rows of c just outside the strip.  EDGE adds the boundary
+    // rule at domain faces; interior waves skip it.
+    template <bool EDGE, int RB>
+    __device__ __forceinline__ void compute(const T (&b)[RB], const T (&c)[RB], const T (&u)[RB], int o, T c_lo,
+                                            T c_hi, int p, int lv, T (&out)[NY]) const
     {
         const bool zlo = EDGE && p == 0;
         const bool zhi = EDGE && p == nz - 1;
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            const T cc = c[r + 1];
+            const T cc = c[o + r];
             T w = dpp_shift0<kDppWaveShr1>(cc);
             T e = dpp_shift0<kDppWaveShl1>(cc);
-            T n = c[r + 2], s = c[r], t = u[r + 1], bt = b[r + 1];
+            T n = r + 1 < NY ? c[o + r + 1] : c_hi;
+            T s = r > 0 ? c[o + r - 1] : c_lo;
+            T t = u[o + r], bt = b[o + r];
             if (EDGE) {
                 const T bv = bc_value<T, BC>(cc);
                 w = xlo ? bv : w;
@@ -158,27 +171,40 @@ struct March {
         }
     }
 
-    __device__ __forceinline__ void level(const T (&b)[R], const T (&c)[R], const T (&u)[R], int p, int lv,
-                                          T (&out)[NY]) const
+    template <int RB>
+    __device__ __forceinline__ void level(const T (&b)[RB], const T (&c)[RB], const T (&u)[RB], int o, T c_lo,
+                                          T c_hi, int p, int lv, T (&out)[NY]) const
     {
-        if (edge || p == 0 || p == nz - 1) compute<true>(b, c, u, p, lv, out);
-        else compute<false>(b, c, u, p, lv, out);
+        if (edge || p == 0 || p == nz - 1) compute<true>(b, c, u, o, c_lo, c_hi, p, lv, out);
+        else compute<false>(b, c, u, o, c_lo, c_hi, p, lv, out);
     }
 
-    // Publishes rows 0 and NY-1 of a level plane and gathers the rows just
-    // outside this wave's strip from the neighbouring waves.
-    __device__ __forceinline__ void exchange(int lv, int par, const T (&out)[NY], T (&dstp)[R])
+    __device__ __forceinline__ T* xrows(int par, int lv) const
     {
-        T* x = xch + ((size_t)(par * (kMaxLevels - 1) + lv - 1) * NW) * 2 * 64;
+        return xch + ((size_t)(par * (kMaxLevels - 1) + lv - 1) * NW) * 2 * 64;
+    }
+
+    // Publishes rows 0 and NY-1 of a level-lv plane (buffer parity par) and
+    // keeps the plane's NY rows in the ring.
+    __device__ __forceinline__ void publish(int lv, int par, const T (&out)[NY], T (&ring)[NY]) const
+    {
+        T* x = xrows(par, lv);
         x[(wave * 2 + 0) * 64 + lane] = out[0];
         x[(wave * 2 + 1) * 64 + lane] = out[NY - 1];
-        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < NY; ++r) ring[r] = out[r];
+    }
+
+    // The rows just below and above this wave's strip of the level-lv plane
+    // published with parity par.  The outermost waves read their own rows:
+    // they are ghosts there.
+    __device__ __forceinline__ void fetch(int lv, int par, T& c_lo, T& c_hi) const
+    {
+        const T* x = xrows(par, lv);
         const int wb = wave > 0 ? wave - 1 : 0;
         const int wa = wave < NW - 1 ? wave + 1 : NW - 1;
-        dstp[0] = x[(wb * 2 + 1) * 64 + lane];
-        dstp[NY + 1] = x[(wa * 2 + 0) * 64 + lane];
-#pragma unroll
-        for (int r = 0; r < NY; ++r) dstp[r + 1] = out[r];
+        c_lo = x[(wb * 2 + 1) * 64 + lane];
+        c_hi = x[(wa * 2 + 0) * 64 + lane];
     }
 
     __device__ __forceinline__ void store(int p, const T (&out)[NY]) const
@@ -198,10 +224,13 @@ struct March {
         if constexpr (LV <= L) {
             const int p = i - LV;
             if (p >= lo[LV] && p < hi[LV]) {
+                T c_lo, c_hi;
+                fetch(LV - 1, (i - 1) & 1, c_lo, c_hi);
                 T out[NY];
-                level(wl[LV - 2][md(U - LV - 1)], wl[LV - 2][md(U - LV)], wl[LV - 2][md(U - LV + 1)], p, LV, out);
+                level(wl[LV - 2][md(U - LV - 1)], wl[LV - 2][md(U - LV)], wl[LV - 2][md(U - LV + 1)], 0, c_lo, c_hi,
+                      p, LV, out);
                 if constexpr (LV == L) store(p, out);
-                else exchange(LV, i & 1, out, wl[LV - 1][md(U - LV)]);
+                else publish(LV, i & 1, out, wl[LV - 1][md(U - LV)]);
             }
             upper<U, LV + 1>(i);
         }
@@ -216,13 +245,14 @@ struct March {
         const int p = i - 1;
         if (p >= lo[1] && p < hi[1]) {
             T out[NY];
-            level(w0[mp(U - 2)], w0[mp(U - 1)], w0[mp(U)], p, 1, out);
+            const T(&c)[R] = w0[mp(U - 1)];
+            level(w0[mp(U - 2)], c, w0[mp(U)], 1, c[0], c[NY + 1], p, 1, out);
             if constexpr (L == 1) store(p, out);
-            else exchange(1, i & 1, out, wl[0][md(U - 1)]);
+            else publish(1, i & 1, out, wl[0][md(U - 1)]);
         }
         upper<U, 2>(i);
+        if constexpr (L > 1) lds_barrier();
     }
-
     template <int U>
     __device__ __forceinline__ bool trip(int i, int iend)
     {
@@ -321,8 +351,8 @@ __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typen
     }
 }
 
-template <int FMT, int BC, int NY, int NW, int P, bool HAS_DIV, bool SR>
-__global__ void __launch_bounds__(NW * 64)
+template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR>
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE)))
 j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S* buf_b, PersistArgs a, KCoefs cf)
 {
     using S = typename Elem<FMT>::S;
@@ -352,25 +382,42 @@ j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S*
 }
 
 // Tile shapes the persistent kernel is built for: rows per wave, waves per
-// workgroup, level-0 ring slots.  Variant 0 is the default; variant 1 keeps
-// one more plane in flight and is selectable through opts->tune[4].
+// workgroup, level-0 ring slots, and the minimum waves per SIMD the register
+// allocation must allow (1: no limit).  The level rings dominate the VGPR
+// count, so the tables differ for FP64 and for FP32 arithmetic:
+//   0  8 waves x 4 rows, 6 slots        (one workgroup per CU)
+//   1  as 0 with 9 slots (one more plane in flight)
+//   2  FP32: as 0 within 128 VGPRs (two workgroups per CU);
+//      FP64: 4 waves x 4 rows (two workgroups per CU, independent barriers)
+//   3  16 waves x 2 rows: the same 64 x 32 tile with twice the waves
+// Selected through opts->tune[4]; the default per format is kDefaultVariant.
 struct Variant {
-    int ny, nw, p;
+    int ny, nw, p, wpe;
 };
-constexpr Variant kVariants[] = {{4, 8, 6}, {4, 8, 9}};
-constexpr int kNumVariants = 2;
+constexpr Variant kVarF32[] = {{4, 8, 6, 1}, {4, 8, 9, 1}, {4, 8, 6, 4}, {2, 16, 6, 1}};
+constexpr Variant kVarF64[] = {{4, 8, 6, 1}, {4, 8, 9, 1}, {4, 4, 6, 1}, {2, 16, 6, 1}};
+constexpr int kNumVariants = 4;
+__host__ __device__ constexpr Variant variant_shape(int fmt, int v)
+{
+    return fmt == PERKS_F64 ? kVarF64[v] : kVarF32[v];
+}
 
 template <int FMT, int BC, bool HAS_DIV, int V, bool SR>
 void* kernel_ptr()
 {
-    constexpr Variant v = kVariants[V];
-    return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, HAS_DIV, SR>);
+    constexpr Variant v = variant_shape(FMT, V);
+    return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR>);
 }
 
 template <int FMT, int BC, bool HAS_DIV, bool SR>
 void* pick_variant(int var)
 {
-    return var == 1 ? kernel_ptr<FMT, BC, HAS_DIV, 1, SR>() : kernel_ptr<FMT, BC, HAS_DIV, 0, SR>();
+    switch (var) {
+    case 1: return kernel_ptr<FMT, BC, HAS_DIV, 1, SR>();
+    case 2: return kernel_ptr<FMT, BC, HAS_DIV, 2, SR>();
+    case 3: return kernel_ptr<FMT, BC, HAS_DIV, 3, SR>();
+    default: return kernel_ptr<FMT, BC, HAS_DIV, 0, SR>();
+    }
 }
 
 template <int FMT, bool SR>

@@ -1,9 +1,12 @@
 // storage16.h -- FP16 / BF16 storage with FP32 arithmetic.
 //
 // Fields are stored as raw 16-bit patterns; every stored value is the FP32
-// result rounded to nearest-even (binary16: the hardware conversion; bfloat16:
-// the bit form below, NaN kept quiet).  Two neighbouring cells travel in one
-// 32-bit word: the cell with the even x index in the low half.
+// result rounded to nearest-even by gfx950's conversions (v_cvt_f16_f32,
+// v_cvt_pk_f16_f32, v_cvt_pk_bf16_f32).  The bfloat16 conversion equals the
+// bit form rne_bf16_bits below on all 2^32 inputs (NaN kept quiet with its
+// payload), which is how the CPU restatement states it.  Two neighbouring
+// cells travel in one 32-bit word: the cell with the even x index in the low
+// half.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -13,6 +16,16 @@
 
 namespace perks {
 
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// Round-to-nearest-even float -> bfloat16 bits, NaN quieted with its payload.
+__host__ __device__ __forceinline__ uint32_t rne_bf16_bits(uint32_t u)
+{
+    return ((u & 0x7FFFFFFFu) > 0x7F800000u) ? ((u >> 16) | 0x40u) : ((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
 template <int FMT>
 struct Store16;
 
@@ -20,15 +33,19 @@ template <>
 struct Store16<PERKS_F16> {
     __device__ static float widen(uint32_t h) { return __half2float(__ushort_as_half((unsigned short)h)); }
     __device__ static uint32_t narrow(float v) { return (uint32_t)__half_as_ushort(__float2half_rn(v)); }
+    __device__ static uint32_t narrow2(float lo, float hi)
+    {
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t{lo, hi}), f16x2_t));
+    }
 };
 
 template <>
 struct Store16<PERKS_BF16> {
     __device__ static float widen(uint32_t h) { return __uint_as_float(h << 16); }
-    __device__ static uint32_t narrow(float v)
+    __device__ static uint32_t narrow(float v) { return narrow2(v, v) & 0xFFFFu; }
+    __device__ static uint32_t narrow2(float lo, float hi)
     {
-        const uint32_t u = __float_as_uint(v);
-        return ((u & 0x7FFFFFFFu) > 0x7F800000u) ? ((u >> 16) | 0x40u) : ((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t{lo, hi}), bf16x2_t));
     }
 };
 
@@ -47,7 +64,7 @@ __device__ __forceinline__ float widen_hi(uint32_t w)
 template <int FMT>
 __device__ __forceinline__ uint32_t pack2(float lo, float hi)
 {
-    return Store16<FMT>::narrow(lo) | (Store16<FMT>::narrow(hi) << 16);
+    return Store16<FMT>::narrow2(lo, hi);
 }
 
 // The value a result takes after a round trip through storage.

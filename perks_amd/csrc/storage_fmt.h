@@ -56,12 +56,7 @@ struct Fmt<bf16s> {
     using C = float;
     __device__ static C ld(const bf16s* p, size_t i) { return __uint_as_float((uint32_t)p[i].v << 16); }
     // Round to nearest even; NaN stays a quiet NaN.
-    __device__ static uint16_t bits(C v)
-    {
-        const uint32_t u = __float_as_uint(v);
-        return ((u & 0x7FFFFFFFu) > 0x7F800000u) ? (uint16_t)((u >> 16) | 0x40u)
-                                                 : (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-    }
+    __device__ static uint16_t bits(C v) { return (uint16_t)Store16<PERKS_BF16>::narrow(v); }
     __device__ static void st(bf16s* p, size_t i, C v) { p[i].v = bits(v); }
     __device__ static C q(C v) { return __uint_as_float((uint32_t)bits(v) << 16); }
     __device__ static C qsr(C v, uint32_t idx, uint32_t key)

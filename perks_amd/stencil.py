@@ -198,7 +198,7 @@ def plan(height, width_y, width_x, dtype="f64", caps: L.DeviceCaps | None = None
     """Pure host planning when `caps` is given.  Returns (code, plan)."""
     o = make_opts(**kw)
     info = L.PlanInfo()
-    dt = L.F64 if dtype in ("f64", L.F64) else L.F32
+    dt = _dt(dtype)
     rc = L.load().perks_j3d_plan(dt, height, width_y, width_x, C.byref(o), C.byref(caps) if caps else None,
                                  int(occupancy), C.byref(info))
     return rc, info
