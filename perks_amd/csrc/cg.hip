@@ -43,7 +43,7 @@ constexpr int kPerLane = 16;
 constexpr int kSeg = kLanes * kPerLane;
 constexpr int kMaxDiag = 16;
 constexpr int kCsrChunk = 8;  // row entries loaded per predicated chunk
-constexpr int kMaxWgPerCu = 4;
+constexpr int kMaxWgPerCu = kCoopMaxWavesPerCu / 4;  // 256-thread workgroups
 
 // Matrix formats the kernel is built for.  DIA7 is the 7-diagonal form with
 // the diagonal count fixed at compile time (every load of a row issued at once).

@@ -122,6 +122,13 @@ __device__ __forceinline__ bool grid_sync(GridSync* gs, unsigned int nblocks, un
     return s_ok != 0;
 }
 
+// Cooperative grids are sized to at most this many waves per CU.  On MI355X
+// a grid of 8 four-wave workgroups per CU (every wave slot the occupancy
+// query reports) did not reach its first barrier within the wait limit,
+// while grids of up to 16 waves per CU meet every barrier; the persistent
+// kernels' measured optimum is at or below that anyway.
+constexpr int kCoopMaxWavesPerCu = 16;
+
 // ---- host side ----------------------------------------------------------------
 enum GridSyncUser { kSyncJ3d = 0, kSyncJ2d = 1, kSyncCg = 2, kSyncUsers = 3 };
 

@@ -26,6 +26,7 @@ struct J2dGrid {
     int steps;
     int tiles;
     int item_end;
+    int seg;  // rows per segment (streaming kernel, tile variant 2)
 };
 
 struct J2dArgs {
@@ -45,10 +46,22 @@ int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o);
 int release_j2d_scratch();
 
 // Kernel entry points per storage type (j2d_<type>.hip); nullptr if not built.
-// tile: 0 or 1 (TileOf in j2d_kernel.h).
+// tile: 0 or 1 (LDS tiles, TileOf in j2d_kernel.h).
 void* j2d_kernel_f64(int shape, int bc, bool has_div, int tile);
 void* j2d_kernel_f32(int shape, int bc, bool has_div, int tile);
 void* j2d_kernel_f16(int shape, int bc, bool has_div, int tile);
 void* j2d_kernel_bf16(int shape, int bc, bool has_div, int tile);
+
+// Deepest pass the row-streaming kernel is built for, by stencil radius.
+__host__ __device__ constexpr int stream_max_levels(int radius)
+{
+    return radius == 1 ? 4 : (radius == 2 ? 2 : 1);
+}
+
+// The row-streaming kernel (tile variant 2, j2d_stream_kernel.h) per storage type.
+void* j2d_stream_kernel_f64(int shape, int bc, bool has_div);
+void* j2d_stream_kernel_f32(int shape, int bc, bool has_div);
+void* j2d_stream_kernel_f16(int shape, int bc, bool has_div);
+void* j2d_stream_kernel_bf16(int shape, int bc, bool has_div);
 
 }  // namespace perks

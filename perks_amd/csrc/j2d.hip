@@ -23,7 +23,11 @@
 // scratch buffers of the batch's sizes, and captures the launch in a hipGraph
 // that later calls replay.
 //
-// The kernel template itself is in j2d_kernel.h.
+// Tile variant 2 is a different kernel (j2d_stream_kernel.h): each wave
+// streams the rows of a 64-column strip through register rings, one per time
+// level, with no LDS; its work items are strips times row segments.
+//
+// The LDS kernel template is in j2d_kernel.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,8 +42,21 @@ namespace perks {
 
 namespace {
 
+constexpr int kStreamTile = 2;     // tile variant of the row-streaming kernel
+constexpr int kStreamWaves = 4;    // its waves per workgroup, one work item each
+constexpr int kStreamItems = 16384;  // work items a launch aims for (segments are cut to reach it)
+
 void* kernel_for(int dtype, int shape, int bc, bool hd, int tile)
 {
+    if (tile == kStreamTile) {
+        switch (dtype) {
+        case PERKS_F64: return j2d_stream_kernel_f64(shape, bc, hd);
+        case PERKS_F32: return j2d_stream_kernel_f32(shape, bc, hd);
+        case PERKS_F16: return j2d_stream_kernel_f16(shape, bc, hd);
+        case PERKS_BF16: return j2d_stream_kernel_bf16(shape, bc, hd);
+        default: return nullptr;
+        }
+    }
     switch (dtype) {
     case PERKS_F64: return j2d_kernel_f64(shape, bc, hd, tile);
     case PERKS_F32: return j2d_kernel_f32(shape, bc, hd, tile);
@@ -49,21 +66,43 @@ void* kernel_for(int dtype, int shape, int bc, bool hd, int tile)
     }
 }
 
-// Tile sizes of the instantiated kernels (j2d_kernel.h TileOf).
+// Tile sizes of the instantiated LDS kernels (j2d_kernel.h TileOf); the
+// streaming kernel's strips are 64 columns wide and its segments are cut per
+// work list.
 void tile_dims(int dtype, int tile, int* tx, int* ty)
 {
+    if (tile == kStreamTile) {
+        *tx = 64;
+        *ty = 0;
+        return;
+    }
     *tx = dtype == PERKS_F64 ? 64 : 128;
     *ty = tile == 1 ? 64 : 32;
 }
 
 int max_levels(int dtype, int tile, int radius)
 {
+    if (tile == kStreamTile)  // registers, and at least 8 stored columns per strip
+        return std::max(1, std::min(stream_max_levels(radius), (64 - 8) / (2 * radius)));
     int tx, ty;
     tile_dims(dtype, tile, &tx, &ty);
     return std::max(1, (std::min(tx, ty) - 2) / (2 * radius));  // at least 2 interior cells per side
 }
 
-int tile_of(const perks_stencil_opts* o) { return o->tune[4] == 1 ? 1 : 0; }
+// opts->tune[4]: 0 = default (streaming), 1 = LDS tile 1, 2 = streaming,
+// 3 = LDS tile 0.  Returned as the internal variant (0, 1: LDS tiles, 2: streaming).
+int tile_of(const perks_stencil_opts* o)
+{
+    switch (o->tune[4]) {
+    case 1: return 1;
+    case 3: return 0;
+    default: return kStreamTile;
+    }
+}
+
+// Workgroups a launch of `items` work items needs (one item per LDS
+// workgroup, one per wave of the streaming kernel).
+int groups_for(int tile, int items) { return tile == kStreamTile ? (items + kStreamWaves - 1) / kStreamWaves : items; }
 
 // A launch configuration: tile variant, time steps per pass, workgroups per
 // CU (0: as many as fit).
@@ -85,8 +124,20 @@ struct Work {
 
 int build_work(const J2dSpec* specs, int n, int R, int tx, int ty, int L, Work* w)
 {
-    const int VW = tx - 2 * R * L, VH = ty - 2 * R * L;
-    if (VW < 1 || VH < 1) return PERKS_ERR_INVALID_ARG;
+    const bool stream = ty == 0;
+    const int VW = tx - 2 * R * L;
+    int VH = ty - 2 * R * L;
+    if (VW < 1 || (!stream && VH < 1)) return PERKS_ERR_INVALID_ARG;
+    if (stream) {
+        // Segment height: enough work items to fill the chip, and at least
+        // 8*R*L rows so that the ghost rows stay a small part of an item.
+        long long strip_rows = 0;
+        for (int i = 0; i < n; ++i)
+            if (specs[i].steps > 0 && specs[i].nx > 0 && specs[i].ny > 0)
+                strip_rows += (long long)((specs[i].nx + VW - 1) / VW) * specs[i].ny;
+        const long long seg = std::max<long long>(8LL * R * L, (strip_rows + kStreamItems - 1) / kStreamItems);
+        VH = (int)std::min<long long>(seg, 0x7fffffffLL);
+    }
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
     auto npass_of = [&](int i) { return specs[i].steps > 0 ? (specs[i].steps + L - 1) / L : 0; };
@@ -111,6 +162,7 @@ int build_work(const J2dSpec* specs, int n, int R, int tx, int ty, int L, Work* 
         g.nx = s.nx;
         g.steps = s.steps;
         g.tiles = (int)tiles;
+        g.seg = stream ? VH : 0;
         w->items += g.tiles;
         g.item_end = w->items;
         w->grids.push_back(g);
@@ -167,6 +219,7 @@ DeviceWork g_work[64];  // per device, for the direct (unplanned) calls
 // Everything a launch needs once the work list is on the device.
 struct Launch {
     void* fn = nullptr;
+    int tile = 0;
     int grid = 0;     // persistent: workgroups; plain: unused (per-step grids from the work list)
     int occ = 0;
     J2dArgs args;
@@ -181,6 +234,7 @@ int prepare_launch(int dtype, const perks_stencil_opts* o, const KCoefs& cf, con
     if (rc != PERKS_OK) return rc;
     out->fn = kernel_for(dtype, o->shape, o->boundary, cf.divisor != 0.0, c.tile);
     if (!out->fn) return PERKS_ERR_UNSUPPORTED;
+    out->tile = c.tile;
     std::memset(&out->args, 0, sizeof(out->args));
     out->args.grids = dw.grids;
     out->args.active = dw.active;
@@ -196,10 +250,11 @@ int prepare_launch(int dtype, const perks_stencil_opts* o, const KCoefs& cf, con
         int occ = 0;
         PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(out->fn), 256, 0));
         if (occ <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
+        occ = std::min(occ, kCoopMaxWavesPerCu / 4);  // 256-thread workgroups
         if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
         if (c.occ > 0 && c.occ < occ) occ = c.occ;
         out->occ = occ;
-        out->grid = std::min(caps.cu_count * occ, w.items);
+        out->grid = std::min(caps.cu_count * occ, groups_for(c.tile, w.items));
         out->args.npass = w.npass;
         out->args.step_offset = 0;
     }
@@ -222,7 +277,7 @@ int enqueue(Launch& l, const perks_stencil_opts* o, const Work& w, hipStream_t s
         for (int s = 0; s < steps; ++s) {
             a.active = l.args.active + s;
             a.step_offset = s;
-            const int g = w.grids[w.active[s] - 1].item_end;
+            const int g = groups_for(l.tile, w.grids[w.active[s] - 1].item_end);
             void* args[] = {&a, &l.cf};
             PERKS_HIP_TRY(hipLaunchKernel(l.fn, dim3(g), dim3(256), args, 0, stream));
         }
@@ -256,7 +311,7 @@ void fill_report(perks_run_report* rep, float ms, const Work& w, const Launch& l
     rep->grid[0] = l.grid > 0 ? l.grid : (w.grids.empty() ? 0 : w.grids.back().item_end);
     rep->grid[1] = rep->grid[2] = 1;
     rep->block = 256;
-    rep->lds_bytes = 2 * tx * ty * (dtype == PERKS_F64 ? 8 : 4);
+    rep->lds_bytes = 2 * tx * ty * (dtype == PERKS_F64 ? 8 : 4);  // 0 for the streaming kernel (ty = 0)
     rep->occupancy = l.occ;
     rep->result_in_b = w.result_in_b.empty() ? 0 : w.result_in_b[0];
     rep->reserved[0] = c.levels;
@@ -461,7 +516,7 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
         Config best_c = p->cfg;
         const int occ_choices[] = {0, 1};
         const int level_choices[] = {1, 2, 3, 4, 6, 8, 12, 16};
-        for (int tile = 0; tile <= 1 && !scratch.empty(); ++tile)
+        for (int tile = 0; tile <= kStreamTile && !scratch.empty(); ++tile)
             for (int L : level_choices)
                 for (int occ : occ_choices) {
                     const Config c{tile, L, occ};

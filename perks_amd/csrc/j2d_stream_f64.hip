@@ -1,0 +1,9 @@
+// j2d_stream_f64.hip -- instantiates the row-streaming 2D Jacobi kernels for f64 storage.
+#include "j2d_stream_kernel.h"
+
+namespace perks {
+void* j2d_stream_kernel_f64(int shape, int bc, bool has_div)
+{
+    return j2ds::sptr<double>(shape, bc, has_div);
+}
+}  // namespace perks

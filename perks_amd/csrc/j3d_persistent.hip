@@ -104,7 +104,8 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     if (o->shape != PERKS_SHAPE_3D_STAR_R1) return PERKS_ERR_UNSUPPORTED;
     if ((long long)ny * nx * elem_bytes(dtype) >= (long long)kOob) return PERKS_ERR_UNSUPPORTED_SIZE;
     if (occupancy <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
-    int occ = occupancy;
+    const Variant v = variant_shape(dtype, variant_of(o));
+    int occ = std::min(occupancy, std::max(1, kCoopMaxWavesPerCu / v.nw));
     if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
     if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
     const int G = caps->cu_count * occ;
@@ -115,7 +116,6 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     out->grid[0] = G;
     out->grid[1] = 1;
     out->grid[2] = 1;
-    const Variant v = variant_shape(dtype, variant_of(o));
     out->block = v.nw * 64;
     out->occupancy = occ;
     out->lds_bytes = (int)(2 * (kMaxLevels - 1) * v.nw * 2 * 64 * (dtype == PERKS_F64 ? 8 : 4));

@@ -1,4 +1,7 @@
-"""2D Jacobi on the GPU through the C ABI, bit for bit against the oracle.
+"""2D Jacobi on the GPU through the C ABI, bit for bit against the oracle:
+the LDS-tile kernels (tile variants 0 and 1; the row-streaming default is
+covered in test_j2d_stream_gpu.py, and the batch plan autotuner here may pick
+any variant).
 
 Every 2D shape of the reference (stars r1..r6, boxes r1 and r2) with its
 reference weights -- including the FP32 box shapes whose centre weight the
@@ -17,6 +20,8 @@ import perks_amd as P
 pytestmark = pytest.mark.gpu
 
 SHAPES = ["2d5pt", "2ds9pt", "2d13pt", "2d17pt", "2d21pt", "2ds25pt", "2d9pt", "2d25pt"]
+LDS = [0, 0, 0, 0, 3]    # opts.tune: LDS tile 128x32 (64x32 FP64)
+LDS1 = [0, 0, 0, 0, 1]   # LDS tile 128x64 (64x64 FP64)
 
 
 def say(*a):
@@ -35,12 +40,13 @@ def check_equal(got, want, what):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("method", ["plain", "persistent"])
 @pytest.mark.parametrize("boundary", ["clamp", "zero"])
-def test_2d_shapes_match_oracle(gpu, shape, dtype, method, boundary):
+@pytest.mark.parametrize("tile", [LDS, LDS1])
+def test_2d_shapes_match_oracle(gpu, shape, dtype, method, boundary, tile):
     dims, steps = (70, 150), 11
-    say(f"[j2d] {shape} {np.dtype(dtype).name} {method} {boundary} dims={dims} steps={steps}")
+    say(f"[j2d] {shape} {np.dtype(dtype).name} {method} {boundary} dims={dims} steps={steps} tile={tile[4]}")
     si = P.shape_info(shape)
     field = O.libc_rand(dims[0] * dims[1], 3, dtype).reshape(dims)
-    got, rep = P.jacobi2d_run(field, steps, shape=shape, method=method, boundary=boundary)
+    got, rep = P.jacobi2d_run(field, steps, shape=shape, method=method, boundary=boundary, tune=tile)
     want = O.iterate(field, O.stencil_from_shape(si, boundary), steps)
     check_equal(got, want, f"{shape} {method} {boundary}")
 
@@ -53,9 +59,8 @@ def test_2d_sizes_and_pass_depths(gpu, dims, steps, depth, dtype):
     coefs = [0.2] * 5
     si = P.shape_info("2d5pt")
     field = O.uniform(dims[0] * dims[1], 4, 0.0, 1.0, dtype).reshape(dims)
-    tune = [0, 0, depth] if depth else None
     got, _ = P.jacobi2d_run(field, steps, shape="2d5pt", method="persistent", boundary="zero", coefs=coefs,
-                            tune=tune)
+                            tune=[0, 0, depth, 0, 3])
     want = O.iterate(field, O.stencil_from_shape(si, "zero", coefs=coefs), steps)
     check_equal(got, want, f"dims {dims}")
 
@@ -79,7 +84,7 @@ def test_2d_reduced_precision_storage(gpu, fmt, shape, method):
     want_init = O.to16(O.uniform(n, 21, -1.0, 1.0, np.float32), fmt).reshape(dims)
     assert np.array_equal(init, want_init), "device 16-bit fill differs from the host sequence"
     rep = P.jacobi2d_device(a, b, dims[0], dims[1], steps, dtype=fmt, shape=shape, method=method, boundary="clamp",
-                            coefs=coefs)
+                            coefs=coefs, tune=LDS)
     got = (b if rep.result_in_b else a).cpu().numpy().view(np.uint16).reshape(dims)
     want = O.iterate16(init, O.stencil_from_shape(si, "clamp", coefs=coefs), steps, fmt)
     check_equal(got, want, f"{fmt} {shape}")
@@ -105,7 +110,7 @@ def test_2d_batch_matches_per_grid_oracle(gpu, dtype, tdt):
         bufs.append((a, b))
         fields.append(f)
     rep, flags = P.jacobi2d_batch([(a, b, wy, wx, st) for (a, b), (wy, wx, st) in zip(bufs, specs)], dtype=dtype,
-                                  shape="2d9pt", method="persistent", boundary="zero", coefs=coefs)
+                                  shape="2d9pt", method="persistent", boundary="zero", coefs=coefs, tune=LDS)
     for i, ((a, b), (wy, wx, st)) in enumerate(zip(bufs, specs)):
         got = (b if flags[i] else a).cpu().numpy()
         want = O.iterate(fields[i], O.stencil_from_shape(si, "zero", coefs=coefs), st)
@@ -131,7 +136,7 @@ def test_2d_matches_committed_reference_gold(gpu, shape, tag, method):
     field = GOLD[f"{shape}_{tag}_in"]
     steps = int(GOLD["steps_2d"])
     say(f"[j2d] reference gold vector {shape} {tag} {method} dims={field.shape} steps={steps}")
-    got, _ = P.jacobi2d_run(field, steps, shape=shape, method=method, boundary="clamp")
+    got, _ = P.jacobi2d_run(field, steps, shape=shape, method=method, boundary="clamp", tune=LDS)
     check_equal(got, GOLD[f"{shape}_{tag}_out"], f"{shape} {tag} {method} vs reference gold")
 
 
@@ -152,7 +157,7 @@ def test_2d_stochastic_rounding_matches_oracle(gpu, fmt, shape, method):
     P.fill_uniform(a, n, 22, -1.0, 1.0, dtype=fmt)
     init = a.cpu().numpy().view(np.uint16).reshape(dims).copy()
     rep = P.jacobi2d_device(a, b, dims[0], dims[1], steps, dtype=fmt, shape=shape, method=method, boundary="clamp",
-                            coefs=coefs, rounding="stochastic", round_seed=77)
+                            coefs=coefs, rounding="stochastic", round_seed=77, tune=LDS)
     got = (b if rep.result_in_b else a).cpu().numpy().view(np.uint16).reshape(dims)
     want = O.iterate16_sr(init, O.stencil_from_shape(si, "clamp", coefs=coefs), steps, fmt, 77)
     check_equal(got, want, f"{fmt} {shape} {method} stochastic")
