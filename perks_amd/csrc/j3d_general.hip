@@ -5,6 +5,12 @@
 // kernel3d_baseline / kernel3d_general paths (stencil/3dstencil/j3d-baseline.cu,
 // j3d-general-kernels.cuh, j3d.cu:129-808); the 7-point shape has its own
 // register-ring kernels (j3d_plain.hip, j3d_persistent_kernel.h).
+//
+// Two kernels serve these shapes: the register-streaming kernel
+// (j3d_stream_kernel.h; one wave per 64 x 8 tile, planes in VGPRs; the
+// default, 1.6-3.7x the other per step at 512^3) and the LDS-plane kernel
+// (j3d_general_kernel.h; a workgroup per 64 x 16/32 tile, planes in LDS).
+// opts->tune[4] picks one: 2 = streaming, 3 = LDS planes, 0 = default.
 #include "j3d_general.h"
 
 #include <algorithm>
@@ -12,6 +18,7 @@
 
 #include "api_common.h"
 #include "j3d_general_kernel.h"
+#include "j3d_stream_kernel.h"
 
 namespace perks {
 
@@ -27,6 +34,26 @@ void* kernel_for(int dtype, int shape, int bc)
     default: return nullptr;
     }
 }
+
+void* stream_kernel_for(int dtype, int shape, int bc)
+{
+    switch (dtype) {
+    case PERKS_F64: return j3s_kernel_f64(shape, bc);
+    case PERKS_F32: return j3s_kernel_f32(shape, bc);
+    case PERKS_F16: return j3s_kernel_f16(shape, bc);
+    case PERKS_BF16: return j3s_kernel_bf16(shape, bc);
+    default: return nullptr;
+    }
+}
+
+bool use_stream(const perks_stencil_opts* o, int radius)
+{
+    if (o->tune[4] == 2) return true;
+    (void)radius;
+    return o->tune[4] != 3;
+}
+
+constexpr int kStreamItems = 16384;  // per-step launches: work items (waves) to aim for
 
 int compute_bytes(int dtype) { return dtype == PERKS_F64 ? 8 : 4; }
 
@@ -72,6 +99,10 @@ int j3g_levels(int dtype, int shape, const perks_stencil_opts* o)
     const ShapeDesc* sd = shape_by_id(shape);
     if (!sd) return 1;
     if (o->method == PERKS_METHOD_PLAIN) return 1;
+    if (use_stream(o, sd->radius)) {
+        const int lm = j3s::stream3_max_levels(sd->radius);
+        return o->tune[2] > 0 ? std::min(o->tune[2], lm) : lm;
+    }
     int tx, ty;
     tile_of(dtype, &tx, &ty);
     const int R = sd->radius;
@@ -90,6 +121,35 @@ int j3g_plan(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, con
     tile_of(dtype, &tx, &ty);
     const int R = sd->radius;
     const int L = j3g_levels(dtype, o->shape, o);
+    if (use_stream(o, R)) {
+        const int VW = 64 - 2 * R * L, VH = j3s::kRows - 2 * R * L;
+        const long long tiles_xy = (long long)((nx + VW - 1) / VW) * ((ny + VH - 1) / VH);
+        std::memset(out, 0, sizeof(*out));
+        out->block = j3s::kWaves * 64;
+        out->tile_x = 64;
+        out->tile_y = j3s::kRows;
+        out->reserved[0] = L;
+        int nzs = 1;
+        long long groups = 0;
+        if (o->method == PERKS_METHOD_PLAIN) {
+            nzs = (int)std::max(1LL, std::min<long long>(nz, (kStreamItems + tiles_xy - 1) / tiles_xy));
+            groups = (tiles_xy * nzs + j3s::kWaves - 1) / j3s::kWaves;
+        } else {
+            if (occupancy <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
+            int occ = std::min(occupancy, kCoopMaxWavesPerCu / j3s::kWaves);
+            if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
+            if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
+            const int G = caps->cu_count * occ;
+            nzs = best_slabs(nz, tiles_xy, G * j3s::kWaves, 2 * R * L);
+            groups = std::min<long long>(G, (tiles_xy * nzs + j3s::kWaves - 1) / j3s::kWaves);
+            out->occupancy = occ;
+        }
+        if (groups > 0x7fffffffLL) return PERKS_ERR_UNSUPPORTED_SIZE;
+        out->grid[0] = (int)groups;
+        out->grid[1] = out->grid[2] = 1;
+        out->reserved[1] = nzs;
+        return PERKS_OK;
+    }
     const int VW = tx - 2 * R * L, VH = ty - 2 * R * L;
     const long long tiles_xy = (long long)((nx + VW - 1) / VW) * ((ny + VH - 1) / VH);
     std::memset(out, 0, sizeof(*out));
@@ -122,16 +182,23 @@ int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, 
 {
     const ShapeDesc* sd = shape_by_id(o->shape);
     if (!sd || sd->dims != 3) return PERKS_ERR_INVALID_ARG;
-    void* fn = kernel_for(dtype, o->shape, o->boundary);
+    const int R = sd->radius;
+    const bool streaming = use_stream(o, R);
+    void* fn = streaming ? stream_kernel_for(dtype, o->shape, o->boundary) : kernel_for(dtype, o->shape, o->boundary);
     if (!fn) return PERKS_ERR_UNSUPPORTED;
     perks_device_caps caps;
     int rc = device_caps_cached(&caps);
     if (rc != PERKS_OK) return rc;
     int tx, ty;
     tile_of(dtype, &tx, &ty);
-    const int R = sd->radius;
+    if (streaming) {
+        tx = 64;
+        ty = j3s::kRows;
+    }
     const int L = j3g_levels(dtype, o->shape, o);
-    const size_t lds = lds_for(dtype, R, L);
+    const size_t lds = streaming ? 0 : lds_for(dtype, R, L);
+    const int threads = streaming ? j3s::kWaves * 64 : j3g::kThreads;
+    const int per_group = streaming ? j3s::kWaves : 1;  // work items per workgroup
     const int VW = tx - 2 * R * L, VH = ty - 2 * R * L;
     const long long tiles_xy = (long long)((nx + VW - 1) / VW) * ((ny + VH - 1) / VH);
 
@@ -161,32 +228,34 @@ int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, 
     const int npass = steps > 0 ? (steps + L - 1) / L : 0;
     PERKS_HIP_TRY(hipEventRecord(ev0, stream));
     if (o->method == PERKS_METHOD_PLAIN) {
-        // One launch per step, every (tile, slab) item its own workgroup; enough
-        // slabs for about eight workgroups per CU.
-        a.nzs = (int)std::max(1LL, std::min<long long>(nz, (8LL * caps.cu_count + tiles_xy - 1) / tiles_xy));
+        // One launch per step.  LDS kernel: every (tile, slab) item its own
+        // workgroup, enough slabs for about eight workgroups per CU; streaming
+        // kernel: one item per wave, enough slabs for kStreamItems items.
+        const long long want = streaming ? kStreamItems : 8LL * caps.cu_count;
+        a.nzs = (int)std::max(1LL, std::min<long long>(nz, (want + tiles_xy - 1) / tiles_xy));
         const long long nitems = tiles_xy * a.nzs;
         if (nitems > 0x7fffffffLL) return PERKS_ERR_UNSUPPORTED_SIZE;
-        grid = (int)nitems;
+        grid = (int)((nitems + per_group - 1) / per_group);
         a.npass = 1;
         for (int s = 0; s < steps; ++s) {
             a.step_offset = s;
             void* args[] = {&a, &kc};
-            PERKS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(j3g::kThreads), args, lds, stream));
+            PERKS_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, stream));
         }
     } else {
-        PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, j3g::kThreads, lds));
+        PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, lds));
         if (occ <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
+        occ = std::min(occ, std::max(1, kCoopMaxWavesPerCu / (threads / 64)));
         if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
         if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
         const int G = caps.cu_count * occ;
-        a.nzs = best_slabs(nz, tiles_xy, G, 2 * R * L);
-        grid = (int)std::min<long long>(G, tiles_xy * a.nzs);
+        a.nzs = best_slabs(nz, tiles_xy, G * per_group, 2 * R * L);
+        grid = (int)std::min<long long>(G, (tiles_xy * a.nzs + per_group - 1) / per_group);
         a.npass = npass;
         a.step_offset = 0;
         void* args[] = {&a, &kc};
         if (steps > 0)
-            PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(j3g::kThreads), args, (unsigned)lds,
-                                                     stream));
+            PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, (unsigned)lds, stream));
     }
     PERKS_HIP_TRY(hipEventRecord(ev1, stream));
     PERKS_HIP_TRY(hipEventSynchronize(ev1));
@@ -204,9 +273,10 @@ int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, 
         rep->gcells_per_s = ms > 0 ? (double)steps * nz * ny * (double)nx / ms / 1e6 : 0.0;
         rep->grid[0] = grid;
         rep->grid[1] = rep->grid[2] = 1;
-        rep->block = j3g::kThreads;
+        rep->block = threads;
         rep->lds_bytes = (int)lds;
-        rep->lds_planes = L * (2 * R + 1);
+        rep->lds_planes = streaming ? 0 : L * (2 * R + 1);
+        rep->reg_planes = streaming ? (L + 1) * (2 * R + 1) : 0;
         rep->occupancy = occ;
         rep->result_in_b = (o->method == PERKS_METHOD_PLAIN ? steps : npass) % 2;
         rep->reserved[0] = L;

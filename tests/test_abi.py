@@ -97,7 +97,7 @@ def test_planner_is_host_only_and_refuses_like_the_reference():
     rc, info = P.plan(64, 64, 64, caps=caps(), occupancy=1, method="persistent", shape="3d27pt")
     assert rc == 0 and info.reserved[0] == 2 and info.tile_x == 64      # shape-generic kernel: 2 steps per pass
     rc, info = P.plan(64, 64, 64, caps=caps(), occupancy=1, method="persistent", shape="3d13pt")
-    assert rc == 0 and info.reserved[0] == 1                            # radius 2: one step per pass fits LDS
+    assert rc == 0 and info.reserved[0] == 1                            # radius 2: one step per pass
     rc, info = P.plan(512, 512, 512, caps=caps(), occupancy=2, method="plain")
     assert rc == 0 and info.grid[0] >= 256
 

@@ -1,6 +1,7 @@
 """Every 3D shape of the reference (7- and 13-point stars, 17-point type-0,
-27-point box, 19-point Poisson) on the shape-generic 3D kernel, bit for bit
-against the oracle and against the reference gold's committed vectors."""
+27-point box, 19-point Poisson) on both shape-generic 3D kernels (register
+streaming and LDS planes), bit for bit against the oracle and against the
+reference gold's committed vectors."""
 import pathlib
 import sys
 
@@ -27,19 +28,23 @@ def check_equal(got, want, what):
         pytest.fail(f"{what}: {len(bad)} cells differ, first {bad[:3].tolist()}")
 
 
-def generic(shape):
-    # the 7-point star has its own kernels; tune[5] = 1 sends it to the generic one
-    return [0, 0, 0, 0, 0, 1] if shape == "3d7pt" else None
+KERNELS = {"stream": 2, "lds": 3}  # opts.tune[4] on the shape-generic path
+
+
+def generic(shape, kernel="stream", depth=0):
+    # the 7-point star has its own kernels; tune[5] = 1 sends it to the generic ones
+    return [0, 0, depth, 0, KERNELS[kernel], 1 if shape == "3d7pt" else 0]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("tag", ["f64", "f32"])
 @pytest.mark.parametrize("method", ["plain", "persistent"])
-def test_3d_shapes_match_committed_reference_gold(gpu, shape, tag, method):
+@pytest.mark.parametrize("kernel", ["stream", "lds"])
+def test_3d_shapes_match_committed_reference_gold(gpu, shape, tag, method, kernel):
     field = GOLD[f"{shape}_{tag}_in"]
     steps = int(GOLD["steps_3d"])
-    say(f"[j3d-shapes] reference gold {shape} {tag} {method} dims={field.shape}")
-    got, _ = P.j3d_run(field, steps, shape=shape, method=method, boundary="clamp", tune=generic(shape))
+    say(f"[j3d-shapes] reference gold {shape} {tag} {method} {kernel} dims={field.shape}")
+    got, _ = P.j3d_run(field, steps, shape=shape, method=method, boundary="clamp", tune=generic(shape, kernel))
     check_equal(got, GOLD[f"{shape}_{tag}_out"], f"{shape} {tag} {method}")
 
 
@@ -47,29 +52,32 @@ def test_3d_shapes_match_committed_reference_gold(gpu, shape, tag, method):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("method", ["plain", "persistent"])
 @pytest.mark.parametrize("boundary", ["clamp", "zero"])
-@pytest.mark.parametrize("dims,steps", [((19, 37, 70), 7), ((6, 5, 130), 3)])
-def test_3d_shapes_match_oracle(gpu, shape, dtype, method, boundary, dims, steps):
-    say(f"[j3d-shapes] {shape} {np.dtype(dtype).name} {method} {boundary} dims={dims} steps={steps}")
+@pytest.mark.parametrize("dims,steps", [((19, 37, 70), 7), ((6, 5, 130), 3), ((3, 1, 1), 4), ((40, 9, 61), 5)])
+@pytest.mark.parametrize("kernel", ["stream", "lds"])
+def test_3d_shapes_match_oracle(gpu, shape, dtype, method, boundary, dims, steps, kernel):
+    say(f"[j3d-shapes] {shape} {np.dtype(dtype).name} {method} {boundary} {kernel} dims={dims} steps={steps}")
     si = P.shape_info(shape)
     field = O.libc_rand(int(np.prod(dims)), 5, dtype).reshape(dims)
-    got, _ = P.j3d_run(field, steps, shape=shape, method=method, boundary=boundary, tune=generic(shape))
+    got, _ = P.j3d_run(field, steps, shape=shape, method=method, boundary=boundary, tune=generic(shape, kernel))
     want = O.iterate(field, O.stencil_from_shape(si, boundary), steps)
     check_equal(got, want, f"{shape} {method} {boundary}")
 
 
 @pytest.mark.parametrize("shape", ["3d13pt", "3d27pt", "poisson"])
 @pytest.mark.parametrize("depth", [1, 2, 3])
-def test_3d_shapes_pass_depths_and_own_weights(gpu, shape, depth):
+@pytest.mark.parametrize("kernel", ["stream", "lds"])
+def test_3d_shapes_pass_depths_and_own_weights(gpu, shape, depth, kernel):
     dims, steps = (23, 30, 100), 8
-    say(f"[j3d-shapes] {shape} depth={depth}")
+    say(f"[j3d-shapes] {shape} depth={depth} {kernel}")
     si = P.shape_info(shape)
     rng = np.random.default_rng(depth)
     c = rng.random(len(si.coef))
     coefs = list(c / c.sum())
     field = O.uniform(int(np.prod(dims)), 9, -1.0, 1.0).reshape(dims)
     got, rep = P.j3d_run(field, steps, shape=shape, method="persistent", boundary="zero", coefs=coefs,
-                         tune=[0, 0, depth])
-    # FP64 tiles are 64 x 16; a level's ring of 2R+1 planes takes 24 KiB (R=1) or 40 KiB (R=2) of the 64 KiB
+                         tune=generic(shape, kernel, depth))
+    # LDS kernel: FP64 tiles are 64 x 16, a level's ring of 2R+1 planes takes 24 KiB (R=1) or 40 KiB
+    # (R=2) of the 64 KiB; streaming kernel: built for 2 levels at radius 1, 1 at radius 2
     assert rep.reserved[0] == (min(depth, 2) if si.radius == 1 else 1)
     want = O.iterate(field, O.stencil_from_shape(si, "zero", coefs=coefs, divisor=0.0), steps)
     check_equal(got, want, f"{shape} depth {depth}")
@@ -79,7 +87,8 @@ def test_3d_shapes_pass_depths_and_own_weights(gpu, shape, depth):
 @pytest.mark.parametrize("fmt", ["bf16", "f16"])
 @pytest.mark.parametrize("method", ["plain", "persistent"])
 @pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
-def test_3d_shapes_16bit_storage(gpu, shape, fmt, method, rounding):
+@pytest.mark.parametrize("kernel", ["stream", "lds"])
+def test_3d_shapes_16bit_storage(gpu, shape, fmt, method, rounding, kernel):
     torch = gpu
     dims, steps = (11, 20, 90), 5
     say(f"[j3d-shapes] {shape} {fmt} {method} {rounding}")
@@ -90,7 +99,7 @@ def test_3d_shapes_16bit_storage(gpu, shape, fmt, method, rounding):
     P.fill_uniform(a, n, 13, 0.0, 1.0, dtype=fmt)
     init = a.cpu().numpy().view(np.uint16).reshape(dims).copy()
     rep = P.j3d_device(a, b, *dims, steps, dtype=fmt, shape=shape, method=method, boundary="clamp",
-                       rounding=rounding, round_seed=21)
+                       rounding=rounding, round_seed=21, tune=generic(shape, kernel))
     got = (b if rep.result_in_b else a).cpu().numpy().view(np.uint16).reshape(dims)
     st = O.stencil_from_shape(si, "clamp")
     want = O.iterate16_sr(init, st, steps, fmt, 21) if rounding == "stochastic" else O.iterate16(init, st, steps, fmt)
