@@ -384,17 +384,18 @@ j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S*
 // Tile shapes the persistent kernel is built for: rows per wave, waves per
 // workgroup, level-0 ring slots, and the minimum waves per SIMD the register
 // allocation must allow (1: no limit).  The level rings dominate the VGPR
-// count, so the tables differ for FP64 and for FP32 arithmetic:
-//   0  8 waves x 4 rows, 6 slots        (one workgroup per CU)
-//   1  as 0 with 9 slots (one more plane in flight)
-//   2  FP32: as 0 within 128 VGPRs (two workgroups per CU);
-//      FP64: 4 waves x 4 rows (two workgroups per CU, independent barriers)
-//   3  16 waves x 2 rows: the same 64 x 32 tile with twice the waves
-// Selected through opts->tune[4]; the default per format is kDefaultVariant.
+// count, so the tables differ for FP64 and for FP32 arithmetic.  Variant 0 is
+// the default (the fastest measured at 512^3):
+//   FP32 arithmetic (FP32, FP16, BF16 storage)    FP64
+//   0  16 waves x 4 rows (64 x 64 tile)            8 waves x 4 rows
+//   1  8 waves x 4 rows, 9 slots                  as 0 with 9 slots
+//   2  8 waves x 4 rows (two workgroups per CU)   4 waves x 4 rows (two per CU)
+//   3  16 waves x 2 rows                          16 waves x 2 rows
+// Selected through opts->tune[4].
 struct Variant {
     int ny, nw, p, wpe;
 };
-constexpr Variant kVarF32[] = {{4, 8, 6, 1}, {4, 8, 9, 1}, {4, 8, 6, 4}, {2, 16, 6, 1}};
+constexpr Variant kVarF32[] = {{4, 16, 6, 1}, {4, 8, 9, 1}, {4, 8, 6, 1}, {2, 16, 6, 1}};
 constexpr Variant kVarF64[] = {{4, 8, 6, 1}, {4, 8, 9, 1}, {4, 4, 6, 1}, {2, 16, 6, 1}};
 constexpr int kNumVariants = 4;
 __host__ __device__ constexpr Variant variant_shape(int fmt, int v)
