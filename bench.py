@@ -305,7 +305,7 @@ def run_j3d_shapes(args):
                    "results": rows},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "j3g_kernel<3d27pt, double>", "algorithmic_bytes_per_launch": alg},
+                     "kernel": "j3s_kernel<3d27pt, double> (register streaming)", "algorithmic_bytes_per_launch": alg},
     }
     print(json.dumps(out), flush=True)
 
@@ -396,7 +396,7 @@ def run_j2d_storage(args):
                    "results": results},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "j2d_kernel<2d9pt, bf16s>", "algorithmic_bytes_per_launch": alg},
+                     "kernel": "j2d_stream_kernel<2d9pt, bf16s>", "algorithmic_bytes_per_launch": alg},
     }
     print(json.dumps(out), flush=True)
 
@@ -485,7 +485,8 @@ def run_batch2d(args):
                    "grids": len(specs), "cell_steps": cell_steps, "results": out_rows},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "j2d_kernel<2d5pt, float> (batched)", "algorithmic_bytes_per_launch": alg},
+                     "kernel": ("j2d_stream_kernel" if r["autotuned_config"]["tile"] == 2 else "j2d_kernel")
+                               + "<2d5pt, float> (batched)", "algorithmic_bytes_per_launch": alg},
     }
     print(json.dumps(out), flush=True)
 
