@@ -99,6 +99,27 @@ struct Elem<PERKS_BF16> : Elem16<PERKS_BF16> {};
 // iteration later.  So the rings keep NY rows per plane, and one workgroup
 // barrier per iteration, after all levels, separates the writes of one
 // parity from its reads and the reads from the next overwrite.
+template <typename T>
+__device__ __forceinline__ T read_lane_t(T v, int l);
+template <>
+__device__ __forceinline__ float read_lane_t<float>(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <>
+__device__ __forceinline__ double read_lane_t<double>(double v, int l)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Side faces: the lanes whose columns lie beyond an x face of the domain hold,
+// at every level, what the boundary rule reads there (0, or the edge column:
+// level 0 through the load, higher levels through side_faces), so waves at
+// an x face take the update without boundary tests.  Waves at a y face and
+// the planes at z faces apply the rule by selects (compute<true>).
 template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
 struct March {
     using E = Elem<FMT>;
@@ -120,7 +141,10 @@ struct March {
     uint32_t soff[NY];
     bool xlo, xhi;
     bool ylo[NY], yhi[NY];
-    bool edge;  // this wave's strip touches an x or y face of the domain
+    bool edge;          // this wave's strip touches a y face of the domain
+    int xcol;           // this lane's column
+    bool xedge;         // some lanes lie beyond a side face
+    int lane_w, lane_e; // lanes of columns 0 and nx-1 (clamped into the wave)
     int lo[L + 1], hi[L + 1];
     int base;
     // stochastic rounding: index of (y0, x) within a plane, cells per plane,
@@ -179,6 +203,22 @@ struct March {
         else compute<false>(b, c, u, o, c_lo, c_hi, p, lv, out);
     }
 
+    // Lanes beyond a side face take the boundary value of their row.
+    __device__ __forceinline__ void side_faces(T (&out)[NY]) const
+    {
+        if (!xedge) return;
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            if constexpr (BC == PERKS_BC_ZERO) {
+                out[r] = (xcol < 0 || xcol > nx - 1) ? (T)0 : out[r];
+            } else {
+                const T vw = read_lane_t<T>(out[r], lane_w);
+                const T ve = read_lane_t<T>(out[r], lane_e);
+                out[r] = xcol < 0 ? vw : (xcol > nx - 1 ? ve : out[r]);
+            }
+        }
+    }
+
     __device__ __forceinline__ T* xrows(int par, int lv) const
     {
         return xch + ((size_t)(par * (kMaxLevels - 1) + lv - 1) * NW) * 2 * 64;
@@ -229,8 +269,12 @@ struct March {
                 T out[NY];
                 level(wl[LV - 2][md(U - LV - 1)], wl[LV - 2][md(U - LV)], wl[LV - 2][md(U - LV + 1)], 0, c_lo, c_hi,
                       p, LV, out);
-                if constexpr (LV == L) store(p, out);
-                else publish(LV, i & 1, out, wl[LV - 1][md(U - LV)]);
+                if constexpr (LV == L) {
+                    store(p, out);
+                } else {
+                    side_faces(out);
+                    publish(LV, i & 1, out, wl[LV - 1][md(U - LV)]);
+                }
             }
             upper<U, LV + 1>(i);
         }
@@ -247,8 +291,12 @@ struct March {
             T out[NY];
             const T(&c)[R] = w0[mp(U - 1)];
             level(w0[mp(U - 2)], c, w0[mp(U)], 1, c[0], c[NY + 1], p, 1, out);
-            if constexpr (L == 1) store(p, out);
-            else publish(1, i & 1, out, wl[0][md(U - 1)]);
+            if constexpr (L == 1) {
+                store(p, out);
+            } else {
+                side_faces(out);
+                publish(1, i & 1, out, wl[0][md(U - 1)]);
+            }
         }
         upper<U, 2>(i);
         if constexpr (L > 1) lds_barrier();
@@ -286,16 +334,21 @@ struct March {
         const int y0 = vy0 - (L - 1) + wave * NY;
         xlo = x == 0;
         xhi = x == nx - 1;
+        xcol = x;
         cbase = (uint32_t)y0 * (uint32_t)nx + (uint32_t)x;
         const bool xout = x >= vx0 && x < vx1;
-        const bool xedge = xw0 <= 0 || xw0 + 63 >= nx - 1;
-        const bool yedge = y0 <= 0 || y0 + NY - 1 >= ny - 1;
-        edge = xedge || yedge;
+        xedge = xw0 < 0 || xw0 + 63 > nx - 1;
+        lane_w = min(max(-xw0, 0), 63);
+        lane_e = min(max(nx - 1 - xw0, 0), 63);
+        edge = y0 <= 0 || y0 + NY - 1 >= ny - 1;
+        // Level 0 outside the domain: the clamped cell (clamp) or 0 (zero, an
+        // out-of-range buffer offset).
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int yy = y0 + r - 1;
             const int yc = yy < 0 ? 0 : (yy >= ny ? ny - 1 : yy);
-            roff[r] = (uint32_t)((yc * nx + xc) * (int)sizeof(S));
+            const bool in = yy >= 0 && yy < ny && x >= 0 && x < nx;
+            roff[r] = (BC == PERKS_BC_ZERO && !in) ? kOob : (uint32_t)((yc * nx + xc) * (int)sizeof(S));
         }
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
