@@ -276,7 +276,8 @@ enum perks_cg_precision { PERKS_CG_F64 = 0, PERKS_CG_F32_REFINE = 1 };
 
 typedef struct perks_cg_opts {
     double tol;          /* stop when r.r < tol^2 (* b.b when relative)      */
-    double inner_tol;    /* F32_REFINE: relative tolerance of each inner solve */
+    double inner_tol;    /* F32_REFINE: relative tolerance of each inner solve; a solve stops
+                            earlier, at sqrt(tol^2 / r.r) / 2, when that is enough for tol */
     int32_t relative;    /* 1: ||r||/||b|| < tol ; 0: ||r|| < tol (cg_kernels.cuh:834) */
     int32_t max_iter;    /* iterations (F64) / per inner solve (F32_REFINE) */
     int32_t precision;   /* enum perks_cg_precision */

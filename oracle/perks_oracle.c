@@ -557,7 +557,11 @@ int po_cg_f32_refine(int fmt, int n, const int32_t* rowptr, const int32_t* col, 
         rr_prev = rr;
         for (int i = 0; i < n; ++i) { bf[i] = (float)r[i]; rf[i] = bf[i]; d[i] = 0.0f; }
         double r1 = po_dot_f32(rf, rf, (size_t)n), r0 = 0.0;
-        const double ithr = inner_tol * inner_tol * po_dot_f32(bf, bf, (size_t)n);
+        /* The inner solve stops at relative tolerance max(inner_tol, sqrt(thr / rr) / 2):
+           never further than the outer target needs. */
+        double tau2 = inner_tol * inner_tol;
+        if (rr > 0.0 && 0.25 * thr / rr > tau2) tau2 = 0.25 * thr / rr;
+        const double ithr = tau2 * po_dot_f32(bf, bf, (size_t)n);
         int k = 1;
         while (!(r1 < ithr) && k <= max_inner) {
             if (k > 1) {

@@ -142,7 +142,8 @@ int po_cg_f64(int fmt, int n, const int32_t* rowptr, const int32_t* col, const d
  * steps, or when r.r repeats; else an FP32 CG (matrix values and r rounded
  * to float, float SpMV/axpy, dot products of exact double products summed in
  * double as po_dot_f32, alpha/beta = (float)(double ratio)) solves A d = r
- * from d = 0 to r_f.r_f < inner_tol^2 * r.r, and x = x + (double)d.
+ * from d = 0 to r_f.r_f < tau^2 * (r_f0.r_f0), tau^2 = max(inner_tol^2,
+ * thr / (4 r.r)) (no further than the outer target needs), and x = x + (double)d.
  * outer_hist[k] = FP64 r.r at outer step k.  Returns total inner iterations;
  * *outer_out = refinement steps taken. */
 int po_cg_f32_refine(int fmt, int n, const int32_t* rowptr, const int32_t* col, const double* val, int ndiag,

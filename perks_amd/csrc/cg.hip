@@ -481,7 +481,8 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
         PERKS_HIP_TRY(hipMemcpy(hist_host.data(), S.hist, sizeof(double) * hist_host.size(), hipMemcpyDeviceToHost));
     } else {
         // Iterative refinement: r = b - A x in FP64, an FP32 CG on A d = r to a
-        // relative tolerance, x += d, until the FP64 residual meets tol.
+        // relative tolerance (inner_tol, relaxed on the last steps to what the
+        // outer target still needs), x += d, until the FP64 residual meets tol.
         const int nb = (int)std::min<size_t>(((size_t)n + 255) / 256, 65536);
         float elapsed = ms;
         double rr_prev = -1.0;
@@ -544,7 +545,10 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
             f.p1 = (float*)S.p1;
             f.ap = (float*)S.ap;
             f.b = S.rf;
+            // The inner solve stops at relative tolerance max(inner_tol, sqrt(thr / rr) / 2):
+            // never further than the outer target needs.
             f.tol2 = o->inner_tol * o->inner_tol;
+            if (rr > 0.0 && 0.25 * thr / rr > f.tol2) f.tol2 = 0.25 * thr / rr;
             f.relative = 1;
             f.max_iter = o->max_inner > 0 ? o->max_inner : o->max_iter;
             f.part_a = S.parts;
