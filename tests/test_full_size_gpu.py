@@ -11,13 +11,16 @@ thing must give the same bits.
 * 512 uneven grids: one captured plan against each grid solved alone.
 
 The small-size tests pin each kernel to the oracle bit for bit; these carry
-that parity to the full sizes.
+that parity to the full sizes.  The 2D 5-point FP64 case at 2048^2 x 100
+steps (the size the reference runs its CPU gold at) is checked against the
+CPU oracle directly.
 """
 import sys
 
 import numpy as np
 import pytest
 
+import oracle as O
 import perks_amd as P
 
 pytestmark = pytest.mark.gpu
@@ -125,3 +128,16 @@ def test_batch_512_grids_plan_equals_each_grid_alone(gpu):
         want = y if rep.result_in_b else x
         assert torch.equal(got, want), f"grid {i} {wy}x{wx} steps {st} differs from its own per-step solve"
     plan.close()
+
+
+@pytest.mark.parametrize("method", ["plain", "persistent"])
+@pytest.mark.parametrize("tile", [2, 3])
+def test_2d5pt_2048_fp64_100_steps_matches_oracle(gpu, method, tile):
+    n, steps = 2048, 100
+    say(f"[full] 2d5pt {n}^2 f64 {steps} steps {method} tile={tile}")
+    coefs = [0.2] * 5
+    field = O.uniform(n * n, 1, 0.0, 1.0).reshape(n, n)
+    got, _ = P.jacobi2d_run(field, steps, shape="2d5pt", method=method, boundary="zero", coefs=coefs,
+                            tune=[0, 0, 0, 0, tile])
+    want = O.iterate(field, O.stencil_from_shape(P.shape_info("2d5pt"), "zero", coefs=coefs), steps, threads=4)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
