@@ -257,15 +257,15 @@ struct March {
     static constexpr int md(int a) { return ((a % 3) + 3) % 3; }
     static constexpr int mp(int a) { return ((a % P) + P) % P; }
 
-    // Levels 2..L of iteration i (stage U = (i - base) mod UF).
+    // Levels 2..L of iteration i (stage U = (i - base) mod UF); hl holds the
+    // halo rows of each level's centre plane, read at the start of the stage.
     template <int U, int LV>
-    __device__ __forceinline__ void upper(int i)
+    __device__ __forceinline__ void upper(int i, const T (&hl)[L > 1 ? L - 1 : 1][2])
     {
         if constexpr (LV <= L) {
             const int p = i - LV;
             if (p >= lo[LV] && p < hi[LV]) {
-                T c_lo, c_hi;
-                fetch(LV - 1, (i - 1) & 1, c_lo, c_hi);
+                const T c_lo = hl[LV - 2][0], c_hi = hl[LV - 2][1];
                 T out[NY];
                 level(wl[LV - 2][md(U - LV - 1)], wl[LV - 2][md(U - LV)], wl[LV - 2][md(U - LV + 1)], 0, c_lo, c_hi,
                       p, LV, out);
@@ -276,7 +276,7 @@ struct March {
                     publish(LV, i & 1, out, wl[LV - 1][md(U - LV)]);
                 }
             }
-            upper<U, LV + 1>(i);
+            upper<U, LV + 1>(i, hl);
         }
     }
 
@@ -286,6 +286,11 @@ struct March {
     __device__ __forceinline__ void stage(int i)
     {
         load0(i + D, mp(U + D));
+        // The halo rows every upper level needs were published last iteration:
+        // request them all now, so they arrive while level 1 computes.
+        T hl[L > 1 ? L - 1 : 1][2];
+#pragma unroll
+        for (int lv = 2; lv <= L; ++lv) fetch(lv - 1, (i - 1) & 1, hl[lv - 2][0], hl[lv - 2][1]);
         const int p = i - 1;
         if (p >= lo[1] && p < hi[1]) {
             T out[NY];
@@ -298,7 +303,7 @@ struct March {
                 publish(1, i & 1, out, wl[0][md(U - 1)]);
             }
         }
-        upper<U, 2>(i);
+        upper<U, 2>(i, hl);
         if constexpr (L > 1) lds_barrier();
     }
     template <int U>
