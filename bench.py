@@ -109,6 +109,23 @@ def _cpu_model() -> str:
     return "unknown CPU"
 
 
+def copy_bandwidth(torch, like) -> float:
+    """GB/s of a device-to-device copy of a field-sized buffer (read + write
+    bytes over the time), timed with HIP events on the current stream."""
+    dst = torch.empty_like(like)
+    dst.copy_(like)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    reps = 5
+    for _ in range(reps):
+        dst.copy_(like)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del dst
+    return 2 * like.numel() * like.element_size() / (ms * 1e-3) / 1e9
+
+
 def load_traffic(n, tsteps, dtype):
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
@@ -176,12 +193,17 @@ def run_j3d(args):
     avg_launch_s = kern_s / args.steps
     alg_bytes = T * cells * 2 * esz
     achieved = alg_bytes / avg_launch_s / 1e9
-    traffic = load_traffic(n, T, args.dtype)
+    traffic = load_traffic(n, T, args.dtype) if args.method == "persistent" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "j3d_star1_persistent_kernel" if args.method == "persistent" else
                           "j3d_star1_plain_kernel",
                 "algorithmic_bytes_per_launch": alg_bytes}
+    # Second denominators: the counter-measured HBM traffic per launch over the
+    # launch time, and what a device-to-device copy of the field achieves here.
+    if traffic:
+        roofline["traffic_gbs"] = round(traffic / avg_launch_s / 1e9, 1)
+    roofline["copy_gbs"] = round(copy_bandwidth(torch, a), 1)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GCells/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(kern_s / args.steps * 1e3, 3),
