@@ -89,7 +89,7 @@ def cpu_baseline(n: int, tsteps: int, dtype: str) -> dict:
         t0 = time.perf_counter()
         O.iterate(field, st, tsteps)
         dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": round(tsteps * n ** 3 / dt / 1e9, 4),
         "unit": "GCells/s",
         "cores": 1,
@@ -97,6 +97,19 @@ def cpu_baseline(n: int, tsteps: int, dtype: str) -> dict:
         "sample": f"{what}; {n}^3 {dtype}, {tsteps} time steps, {dt:.1f} s on {os.cpu_count()} host CPUs "
                   f"({_cpu_model()}), one thread used",
     }
+    # The same sample on every host core: the oracle's OpenMP restatement
+    # (bit-identical to the single-threaded one), capped at the 16 CPUs a
+    # one-GPU share of the host provides.
+    import perks_amd as P
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    st = O.stencil_from_shape(P.shape_info("3d_star_r1"), "zero", coefs=CONFIG_COEFS)
+    t0 = time.perf_counter()
+    O.iterate(field, st, tsteps, threads=threads)
+    dtm = time.perf_counter() - t0
+    out["all_cores"] = {"value": round(tsteps * n ** 3 / dtm / 1e9, 4), "unit": "GCells/s", "cores": threads,
+                        "kind": "port", "sample": f"oracle po_stencil_iterate_omp ({threads} threads); {n}^3 {dtype}, "
+                                                  f"{tsteps} time steps, {dtm:.2f} s"}
+    return out
 
 
 def _cpu_model() -> str:
