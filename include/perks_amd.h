@@ -101,8 +101,10 @@ typedef struct perks_stencil_opts {
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
      *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel).
      *           3D other shapes: 0 = default (2), 2 = register streaming, 3 = LDS planes.
-     *           2D: 0 = default (2), 1 = LDS tile 128x64 (64x64 FP64), 2 = row streaming
-     *           through registers, 3 = LDS tile 128x32 (64x32 FP64)
+     *           2D: 0 = default (4 where it applies, else 2), 1 = LDS tile 128x64 (64x64 FP64),
+     *           2 = row streaming through registers, one column per lane, 3 = LDS tile 128x32
+     *           (64x32 FP64), 4 = row streaming, two columns per lane (FP32/FP16/BF16 storage,
+     *           even widths; otherwise PERKS_ERR_UNSUPPORTED)
      *   tune[5] 3D: 1 = run the 7-point star on the shape-generic kernel
      *   tune[6..7] unused */
     int32_t tune[8];

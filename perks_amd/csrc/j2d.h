@@ -42,7 +42,7 @@ struct J2dArgs {
 };
 
 int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);
-int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o);
+int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o, int tile);
 int release_j2d_scratch();
 
 // Kernel entry points per storage type (j2d_<type>.hip); nullptr if not built.
@@ -58,10 +58,15 @@ __host__ __device__ constexpr int stream_max_levels(int radius)
     return radius == 1 ? 4 : (radius == 2 ? 2 : 1);
 }
 
-// The row-streaming kernel (tile variant 2, j2d_stream_kernel.h) per storage type.
-void* j2d_stream_kernel_f64(int shape, int bc, bool has_div);
-void* j2d_stream_kernel_f32(int shape, int bc, bool has_div);
-void* j2d_stream_kernel_f16(int shape, int bc, bool has_div);
-void* j2d_stream_kernel_bf16(int shape, int bc, bool has_div);
+// Ghost columns per side of a two-column streaming strip: R*L rounded up to even.
+__host__ __device__ constexpr int pair_ghost(int radius, int levels) { return 2 * ((radius * levels + 1) / 2); }
+
+// The row-streaming kernel (j2d_stream_kernel.h) per storage type: tile
+// variant 2 (one column per lane) or, with pair, 4 (two columns per lane;
+// FP32/FP16/BF16 storage and even grid widths; nullptr for FP64).
+void* j2d_stream_kernel_f64(int shape, int bc, bool has_div, bool pair);
+void* j2d_stream_kernel_f32(int shape, int bc, bool has_div, bool pair);
+void* j2d_stream_kernel_f16(int shape, int bc, bool has_div, bool pair);
+void* j2d_stream_kernel_bf16(int shape, int bc, bool has_div, bool pair);
 
 }  // namespace perks

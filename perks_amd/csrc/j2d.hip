@@ -23,9 +23,12 @@
 // scratch buffers of the batch's sizes, and captures the launch in a hipGraph
 // that later calls replay.
 //
-// Tile variant 2 is a different kernel (j2d_stream_kernel.h): each wave
-// streams the rows of a 64-column strip through register rings, one per time
-// level, with no LDS; its work items are strips times row segments.
+// Tile variants 2 and 4 are a different kernel (j2d_stream_kernel.h): each
+// wave streams the rows of a strip through register rings, one per time
+// level, with no LDS; its work items are strips times row segments.  Variant
+// 2 holds one column per lane (64-column strips), variant 4 two (128-column
+// strips, FP32/FP16/BF16 storage and even grid widths); the default is 4
+// where it applies, else 2.
 //
 // The LDS kernel template is in j2d_kernel.h.
 #include <hip/hip_runtime.h>
@@ -42,18 +45,21 @@ namespace perks {
 
 namespace {
 
-constexpr int kStreamTile = 2;     // tile variant of the row-streaming kernel
+constexpr int kStreamTile = 2;     // tile variant of the row-streaming kernel, one column per lane
+constexpr int kPairTile = 4;       // the row-streaming kernel, two columns per lane
+constexpr int kDefaultTile = -1;   // resolved per run: kPairTile where it applies, else kStreamTile
 constexpr int kStreamWaves = 4;    // its waves per workgroup, one work item each
 constexpr int kStreamItems = 16384;  // work items a launch aims for (segments are cut to reach it)
 
 void* kernel_for(int dtype, int shape, int bc, bool hd, int tile)
 {
-    if (tile == kStreamTile) {
+    if (tile == kStreamTile || tile == kPairTile) {
+        const bool pair = tile == kPairTile;
         switch (dtype) {
-        case PERKS_F64: return j2d_stream_kernel_f64(shape, bc, hd);
-        case PERKS_F32: return j2d_stream_kernel_f32(shape, bc, hd);
-        case PERKS_F16: return j2d_stream_kernel_f16(shape, bc, hd);
-        case PERKS_BF16: return j2d_stream_kernel_bf16(shape, bc, hd);
+        case PERKS_F64: return j2d_stream_kernel_f64(shape, bc, hd, pair);
+        case PERKS_F32: return j2d_stream_kernel_f32(shape, bc, hd, pair);
+        case PERKS_F16: return j2d_stream_kernel_f16(shape, bc, hd, pair);
+        case PERKS_BF16: return j2d_stream_kernel_bf16(shape, bc, hd, pair);
         default: return nullptr;
         }
     }
@@ -71,8 +77,8 @@ void* kernel_for(int dtype, int shape, int bc, bool hd, int tile)
 // work list.
 void tile_dims(int dtype, int tile, int* tx, int* ty)
 {
-    if (tile == kStreamTile) {
-        *tx = 64;
+    if (tile == kStreamTile || tile == kPairTile) {
+        *tx = tile == kPairTile ? 128 : 64;
         *ty = 0;
         return;
     }
@@ -84,25 +90,57 @@ int max_levels(int dtype, int tile, int radius)
 {
     if (tile == kStreamTile)  // registers, and at least 8 stored columns per strip
         return std::max(1, std::min(stream_max_levels(radius), (64 - 8) / (2 * radius)));
+    if (tile == kPairTile) {
+        int l = stream_max_levels(radius);
+        while (l > 1 && 128 - 2 * pair_ghost(radius, l) < 8) --l;
+        return l;
+    }
     int tx, ty;
     tile_dims(dtype, tile, &tx, &ty);
     return std::max(1, (std::min(tx, ty) - 2) / (2 * radius));  // at least 2 interior cells per side
 }
 
-// opts->tune[4]: 0 = default (streaming), 1 = LDS tile 1, 2 = streaming,
-// 3 = LDS tile 0.  Returned as the internal variant (0, 1: LDS tiles, 2: streaming).
+// opts->tune[4]: 0 = default, 1 = LDS tile 1, 2 = streaming (one column per
+// lane), 3 = LDS tile 0, 4 = streaming (two columns per lane).  Returned as
+// the internal variant (0, 1: LDS tiles, 2, 4: streaming; kDefaultTile).
 int tile_of(const perks_stencil_opts* o)
 {
     switch (o->tune[4]) {
     case 1: return 1;
+    case 2: return kStreamTile;
     case 3: return 0;
-    default: return kStreamTile;
+    case 4: return kPairTile;
+    default: return kDefaultTile;
     }
+}
+
+// The two-column kernel needs FP32/FP16/BF16 storage and even widths.
+bool pair_applies(int dtype, const J2dSpec* specs, int n)
+{
+    if (dtype == PERKS_F64) return false;
+    for (int i = 0; i < n; ++i)
+        if (specs[i].steps > 0 && (specs[i].nx & 1)) return false;
+    return true;
+}
+
+// The variant a run uses: the requested one, the default resolved, or an
+// error when the two-column kernel was asked for where it does not apply.
+int resolve_tile(int dtype, const perks_stencil_opts* o, const J2dSpec* specs, int n, int* tile)
+{
+    int t = tile_of(o);
+    const bool pair_ok = pair_applies(dtype, specs, n);
+    if (t == kDefaultTile) t = pair_ok ? kPairTile : kStreamTile;
+    if (t == kPairTile && !pair_ok) return PERKS_ERR_UNSUPPORTED;
+    *tile = t;
+    return PERKS_OK;
 }
 
 // Workgroups a launch of `items` work items needs (one item per LDS
 // workgroup, one per wave of the streaming kernel).
-int groups_for(int tile, int items) { return tile == kStreamTile ? (items + kStreamWaves - 1) / kStreamWaves : items; }
+int groups_for(int tile, int items)
+{
+    return (tile == kStreamTile || tile == kPairTile) ? (items + kStreamWaves - 1) / kStreamWaves : items;
+}
 
 // A launch configuration: tile variant, time steps per pass, workgroups per
 // CU (0: as many as fit).
@@ -125,7 +163,7 @@ struct Work {
 int build_work(const J2dSpec* specs, int n, int R, int tx, int ty, int L, Work* w)
 {
     const bool stream = ty == 0;
-    const int VW = tx - 2 * R * L;
+    const int VW = (stream && tx == 128) ? 128 - 2 * pair_ghost(R, L) : tx - 2 * R * L;
     int VH = ty - 2 * R * L;
     if (VW < 1 || (!stream && VH < 1)) return PERKS_ERR_INVALID_ARG;
     if (stream) {
@@ -321,14 +359,11 @@ void fill_report(perks_run_report* rep, float ms, const Work& w, const Launch& l
 
 }  // namespace
 
-int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o)
+int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o, int tile)
 {
     const ShapeDesc* sd = shape_by_id(shape);
     if (!sd) return 1;
     if (o->method == PERKS_METHOD_PLAIN) return 1;
-    const int tile = tile_of(o);
-    int tx, ty;
-    tile_dims(dtype, tile, &tx, &ty);
     // Default: 4 / R steps per pass (the best depths measured at 8192^2 for
     // radius 1, 2 and 3-6 on the default tile).
     const int L = o->tune[2] > 0 ? o->tune[2] : std::max(1, 4 / sd->radius);
@@ -345,7 +380,10 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     KCoefs cf;
     rc = resolve_coefs(o, &cf, nullptr);
     if (rc != PERKS_OK) return rc;
-    const Config c{tile_of(o), j2d_pass_levels(dtype, o->shape, o), o->tune[3]};
+    int tile = 0;
+    rc = resolve_tile(dtype, o, specs, n, &tile);
+    if (rc != PERKS_OK) return rc;
+    const Config c{tile, j2d_pass_levels(dtype, o->shape, o, tile), o->tune[3]};
     int tx, ty;
     tile_dims(dtype, c.tile, &tx, &ty);
     Work w;
@@ -487,7 +525,13 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
         delete p;
         return rc;
     }
-    p->cfg = Config{tile_of(o), j2d_pass_levels(dtype, o->shape, o), o->tune[3]};
+    int tile = 0;
+    rc = resolve_tile(dtype, o, specs.data(), ngrids, &tile);
+    if (rc != PERKS_OK) {
+        delete p;
+        return rc;
+    }
+    p->cfg = Config{tile, j2d_pass_levels(dtype, o->shape, o, tile), o->tune[3]};
 
     if (autotune && o->method != PERKS_METHOD_PLAIN) {
         // Scratch copies of (a prefix of) the batch's grids; contents do not matter for timing.
@@ -516,10 +560,12 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
         Config best_c = p->cfg;
         const int occ_choices[] = {0, 1};
         const int level_choices[] = {1, 2, 3, 4, 6, 8, 12, 16};
-        for (int tile = 0; tile <= kStreamTile && !scratch.empty(); ++tile)
+        std::vector<int> tiles = {0, 1, kStreamTile};
+        if (pair_applies(dtype, specs.data(), ngrids)) tiles.push_back(kPairTile);
+        for (int tl : tiles)
             for (int L : level_choices)
                 for (int occ : occ_choices) {
-                    const Config c{tile, L, occ};
+                    const Config c{tl, L, occ};
                     const double v = time_config(dtype, o, cf, c, scratch, sd->radius, kTuneSteps, stream, p->gs);
                     if (v > best) {
                         best = v;

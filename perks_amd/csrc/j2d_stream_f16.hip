@@ -2,8 +2,8 @@
 #include "j2d_stream_kernel.h"
 
 namespace perks {
-void* j2d_stream_kernel_f16(int shape, int bc, bool has_div)
+void* j2d_stream_kernel_f16(int shape, int bc, bool has_div, bool pair)
 {
-    return j2ds::sptr<j2dk::f16s>(shape, bc, has_div);
+    return j2ds::sptr<j2dk::f16s>(shape, bc, has_div, pair);
 }
 }  // namespace perks

@@ -238,7 +238,286 @@ struct Strip {
     }
 };
 
-template <int SHAPE, typename S, int BC, bool HAS_DIV>
+// ---- two columns per lane ---------------------------------------------------------
+// Storage access to the cells of columns x0 (even) and x0 + 1 in one load or
+// store, and their rounding to the storage format.
+template <typename S>
+struct Pair2;
+template <>
+struct Pair2<float> {
+    __device__ static void ld(const float* row, size_t x0, float& a, float& b)
+    {
+        const float2 v = *reinterpret_cast<const float2*>(row + x0);
+        a = v.x;
+        b = v.y;
+    }
+    __device__ static void st(float* row, size_t x0, float a, float b)
+    {
+        *reinterpret_cast<float2*>(row + x0) = make_float2(a, b);
+    }
+    __device__ static void q2(float&, float&, bool, uint32_t, uint32_t) {}
+};
+template <typename S16, int FMT>
+struct Pair16 {
+    __device__ static void ld(const S16* row, size_t x0, float& a, float& b)
+    {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(row + x0);
+        a = widen_lo<FMT>(w);
+        b = widen_hi<FMT>(w);
+    }
+    // a and b are already storage values: the conversion is exact.
+    __device__ static void st(S16* row, size_t x0, float a, float b)
+    {
+        *reinterpret_cast<uint32_t*>(row + x0) = Store16<FMT>::narrow2(a, b);
+    }
+    __device__ static void q2(float& a, float& b, bool sr, uint32_t idx0, uint32_t key)
+    {
+        if (sr) {
+            a = Fmt<S16>::qsr(a, idx0, key);
+            b = Fmt<S16>::qsr(b, idx0 + 1u, key);
+        } else {
+            const uint32_t w = Store16<FMT>::narrow2(a, b);
+            a = widen_lo<FMT>(w);
+            b = widen_hi<FMT>(w);
+        }
+    }
+};
+template <>
+struct Pair2<fmt::f16s> : Pair16<fmt::f16s, PERKS_F16> {};
+template <>
+struct Pair2<fmt::bf16s> : Pair16<fmt::bf16s, PERKS_BF16> {};
+
+// v moved by s lanes (lane i receives lane i + s), s static after unrolling.
+template <int R, typename C>
+__device__ __forceinline__ C lane_shift(C v, int s)
+{
+    C w = v;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (s < 0 && k < -s) w = dpp_shift0<kDppWaveShr1>(w);
+        if (s > 0 && k < s) w = dpp_shift0<kDppWaveShl1>(w);
+    }
+    return w;
+}
+
+// The pair (columns x0 + dx, x1 + dx) for dx in [-R, R] from the pair (a, b)
+// of columns (x0, x1): column x0 + m is lane floor(m/2), half m mod 2.
+template <int R, typename C>
+__device__ __forceinline__ void pair_shifts(C a, C b, C (&sa)[2 * R + 1], C (&sb)[2 * R + 1])
+{
+#pragma unroll
+    for (int dx = -R; dx <= R; ++dx) {
+        const int m0 = dx, m1 = dx + 1;
+        const int s0 = m0 >= 0 ? m0 / 2 : -((1 - m0) / 2);
+        const int s1 = m1 >= 0 ? m1 / 2 : -((1 - m1) / 2);
+        sa[R + dx] = lane_shift<R, C>((m0 - 2 * s0) ? b : a, s0);
+        sb[R + dx] = lane_shift<R, C>((m1 - 2 * s1) ? b : a, s1);
+    }
+}
+
+// As Strip, with each lane holding two neighbouring columns (a: even x0,
+// b: x0 + 1): one load, store and set of lane shifts serves two cells.  The
+// strip is 128 columns wide and starts on an even column; the grid width must
+// be even.
+template <int SHAPE, typename S, int BC, bool HAS_DIV, int L>
+struct Strip2 {
+    using C = typename Fmt<S>::C;
+    static constexpr const ShapeDesc& SD = shape_desc<SHAPE>();
+    static constexpr int R = SD.radius;
+    static constexpr bool BOX = SD.is_box != 0;
+    static constexpr int Q = 2 * R + 1;
+    static constexpr int D = Q * ((8 + Q - 1) / Q);
+    static constexpr int P = Q + D;
+    static constexpr int UF = P;
+    static constexpr int NS = BOX ? Q : 1;
+
+    const S* src;
+    S* dst;
+    int nx, ny;
+    int x0;              // this lane's even column
+    bool pout, xedge;    // pair outside the grid; strip reaches past a side face
+    int lane_w, lane_e;  // lanes holding columns 0 (half a) and nx-1 (half b)
+    bool st_ok;
+    int lo[L + 1], hi[L + 1];
+    int ys, ye;
+    C k[kMaxGroups];
+    double kd[kMaxGroups];
+    double kdiv;
+    bool sr;
+    uint32_t key[L + 1];
+
+    C raw[P][2];
+    C win[L][Q][NS][2];
+
+    static constexpr int mp(int v) { return ((v % P) + P) % P; }
+    static constexpr int mq(int v) { return ((v % Q) + Q) % Q; }
+
+    __device__ __forceinline__ void load(int q, C (&out)[2]) const
+    {
+        const int qc = q < 0 ? 0 : (q >= ny ? ny - 1 : q);
+        const int xc = x0 < 0 ? 0 : (x0 > nx - 2 ? nx - 2 : x0);
+        C a, b;
+        Pair2<S>::ld(src + (size_t)qc * nx, (size_t)xc, a, b);
+        if constexpr (BC == PERKS_BC_ZERO) {
+            const bool z = q < 0 || q >= ny || pout;
+            out[0] = z ? (C)0 : a;
+            out[1] = z ? (C)0 : b;
+        } else {
+            out[0] = x0 > nx - 2 ? b : a;  // right of the grid: column nx-1 twice
+            out[1] = x0 < 0 ? a : b;       // left of the grid: column 0 twice
+        }
+    }
+
+    template <int LV>
+    __device__ __forceinline__ C cell(int rel, int dx, int h, const C (&csa)[Q], const C (&csb)[Q], int centre) const
+    {
+        if constexpr (BOX) return win[LV][mq(rel)][R + dx][h];
+        else {
+            if (rel == centre) return h ? csb[R + dx] : csa[R + dx];
+            if constexpr (LV == 0) return raw[mp(rel)][h];
+            else return win[LV][mq(rel)][0][h];
+        }
+    }
+
+    __device__ __forceinline__ void side_faces(C& a, C& b) const
+    {
+        if (!xedge) return;
+        if constexpr (BC == PERKS_BC_ZERO) {
+            a = pout ? (C)0 : a;
+            b = pout ? (C)0 : b;
+        } else {
+            const C vw = read_lane(a, lane_w);
+            const C ve = read_lane(b, lane_e);
+            a = x0 < 0 ? vw : (x0 > nx - 2 ? ve : a);
+            b = x0 < 0 ? vw : (x0 > nx - 2 ? ve : b);
+        }
+    }
+
+    template <int U, int LV>
+    __device__ __forceinline__ void level(int i)
+    {
+        if constexpr (LV < L) {
+            constexpr int LV1 = LV + 1;
+            constexpr int REL = U - R * LV1;
+            const int q = i - R * LV1;
+            C va = (C)0, vb = (C)0;
+            if (q >= lo[LV1] && q < hi[LV1] && q < ny) {
+                C csa[Q], csb[Q];
+                if constexpr (!BOX) {
+                    C ca, cb;
+                    if constexpr (LV == 0) {
+                        ca = raw[mp(REL)][0];
+                        cb = raw[mp(REL)][1];
+                    } else {
+                        ca = win[LV][mq(REL)][0][0];
+                        cb = win[LV][mq(REL)][0][1];
+                    }
+                    pair_shifts<R>(ca, cb, csa, csb);
+                }
+                va = eval_shape<SHAPE, C, HAS_DIV>(k, kdiv, kd, [&](int dy, int dx) -> C {
+                    return cell<LV>(REL + dy, dx, 0, csa, csb, REL);
+                });
+                vb = eval_shape<SHAPE, C, HAS_DIV>(k, kdiv, kd, [&](int dy, int dx) -> C {
+                    return cell<LV>(REL + dy, dx, 1, csa, csb, REL);
+                });
+                Pair2<S>::q2(va, vb, sr, (uint32_t)q * (uint32_t)nx + (uint32_t)x0, key[LV1]);
+                if constexpr (LV1 == L) {
+                    if (st_ok) Pair2<S>::st(dst + (size_t)q * nx, (size_t)x0, va, vb);
+                }
+            }
+            if constexpr (LV1 < L) {
+                side_faces(va, vb);
+                C row[NS][2];
+                if constexpr (BOX) {
+                    C sa[Q], sb[Q];
+                    pair_shifts<R>(va, vb, sa, sb);
+#pragma unroll
+                    for (int t = 0; t < Q; ++t) {
+                        row[t][0] = sa[t];
+                        row[t][1] = sb[t];
+                    }
+                } else {
+                    row[0][0] = va;
+                    row[0][1] = vb;
+                }
+                const int jb = q - (ny - 1);
+#pragma unroll
+                for (int jj = 1; jj <= R; ++jj)
+#pragma unroll
+                    for (int t = 0; t < NS; ++t)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+                            row[t][h] = jb == jj ? (BC == PERKS_BC_ZERO ? (C)0 : win[LV1][mq(REL - jj)][t][h])
+                                                 : row[t][h];
+#pragma unroll
+                for (int t = 0; t < NS; ++t) {
+                    win[LV1][mq(REL)][t][0] = row[t][0];
+                    win[LV1][mq(REL)][t][1] = row[t][1];
+                }
+#pragma unroll
+                for (int j = 1; j <= R; ++j)
+#pragma unroll
+                    for (int t = 0; t < NS; ++t)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+                            win[LV1][mq(REL - j)][t][h] =
+                                q == 0 ? (BC == PERKS_BC_ZERO ? (C)0 : row[t][h]) : win[LV1][mq(REL - j)][t][h];
+            }
+            level<U, LV + 1>(i);
+        }
+    }
+
+    template <int U>
+    __device__ __forceinline__ void stage(int i)
+    {
+        load(i + D, raw[mp(U + D)]);
+        if constexpr (BOX) {
+            C sa[Q], sb[Q];
+            pair_shifts<R>(raw[mp(U)][0], raw[mp(U)][1], sa, sb);
+#pragma unroll
+            for (int t = 0; t < Q; ++t) {
+                win[0][mq(U)][t][0] = sa[t];
+                win[0][mq(U)][t][1] = sb[t];
+            }
+        }
+        level<U, 0>(i);
+    }
+
+    template <int U>
+    __device__ __forceinline__ bool trip(int i, int iend)
+    {
+        if constexpr (U < UF) {
+            if (i > iend) return false;
+            stage<U>(i);
+            return trip<U + 1>(i + 1, iend);
+        }
+        return true;
+    }
+
+    // Strip whose lane 0 holds columns xs0, xs0+1 (xs0 even); stores columns [vx0, vx1).
+    __device__ void run(int lane, int xs0, int vx0, int vx1)
+    {
+        x0 = xs0 + 2 * lane;
+        pout = x0 < 0 || x0 > nx - 2;
+        xedge = xs0 < 0 || xs0 + 127 > nx - 1;
+        lane_w = min(max(-xs0 / 2, 0), 63);
+        lane_e = min(max((nx - 2 - xs0) / 2, 0), 63);
+        st_ok = x0 >= vx0 && x0 < vx1;
+#pragma unroll
+        for (int l = 0; l <= L; ++l) {
+            lo[l] = max(0, ys - R * (L - l));
+            hi[l] = l == L ? ye : min(ny + R, ye + R * (L - l));
+        }
+        const int i0 = ys - R * L;
+        const int iend = ye - 1 + R * L;
+#pragma unroll
+        for (int t = 0; t < D; ++t) load(i0 + t, raw[t]);
+        for (int i = i0; trip<0>(i, iend); i += UF) {
+        }
+    }
+};
+
+template <int SHAPE, typename S, int BC, bool HAS_DIV, bool PAIR>
 __global__ void __launch_bounds__(kWaves * 64) j2d_stream_kernel(J2dArgs a, KCoefs cf)
 {
     using C = typename Fmt<S>::C;
@@ -262,8 +541,8 @@ __global__ void __launch_bounds__(kWaves * 64) j2d_stream_kernel(J2dArgs a, KCoe
         const int gp = pass + a.step_offset;
         const int ngrids = a.active[pass];
         const int nitems = a.grids[ngrids - 1].item_end;
-        const int RL = R * a.levels;  // strips use the full-pass ghost rim in every pass
-        const int VW = 64 - 2 * RL;
+        // Strips use the full-pass ghost rim in every pass.
+        const int VW = PAIR ? 128 - 2 * pair_ghost(R, a.levels) : 64 - 2 * R * a.levels;
         for (int item = me; item < nitems; item += W) {
             int lo = 0, hi = ngrids - 1;  // first grid whose item_end > item
             while (lo < hi) {
@@ -281,7 +560,8 @@ __global__ void __launch_bounds__(kWaves * 64) j2d_stream_kernel(J2dArgs a, KCoe
             S* dst = (S*)((gp & 1) ? gd.a : gd.b);
             auto go = [&](auto lv) {
                 constexpr int LL = decltype(lv)::value;
-                Strip<SHAPE, S, BC, HAS_DIV, LL> s;
+                using St = std::conditional_t<PAIR, Strip2<SHAPE, S, BC, HAS_DIV, LL>, Strip<SHAPE, S, BC, HAS_DIV, LL>>;
+                St s;
                 s.src = src;
                 s.dst = dst;
                 s.nx = gd.nx;
@@ -298,7 +578,7 @@ __global__ void __launch_bounds__(kWaves * 64) j2d_stream_kernel(J2dArgs a, KCoe
 #pragma unroll
                 for (int l = 1; l <= LL; ++l)
                     s.key[l] = a.sr ? sr_key((uint32_t)(gp * a.levels + l - 1), a.seed) : 0u;
-                s.run(lane, vx0 - R * LL, vx0, vx1);
+                s.run(lane, vx0 - (PAIR ? pair_ghost(R, LL) : R * LL), vx0, vx1);
             };
             if (L <= 1 || LM < 2) {
                 go(std::integral_constant<int, 1>{});
@@ -314,29 +594,40 @@ __global__ void __launch_bounds__(kWaves * 64) j2d_stream_kernel(J2dArgs a, KCoe
     }
 }
 
-template <int SHAPE, typename S>
+template <int SHAPE, typename S, bool PAIR>
 void* sptr2(int bc, bool hd)
 {
     if (bc == PERKS_BC_ZERO)
-        return hd ? reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_ZERO, true>)
-                  : reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_ZERO, false>);
-    return hd ? reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_CLAMP, true>)
-              : reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_CLAMP, false>);
+        return hd ? reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_ZERO, true, PAIR>)
+                  : reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_ZERO, false, PAIR>);
+    return hd ? reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_CLAMP, true, PAIR>)
+              : reinterpret_cast<void*>(&j2d_stream_kernel<SHAPE, S, PERKS_BC_CLAMP, false, PAIR>);
 }
 
-template <typename S>
-void* sptr(int shape, int bc, bool hd)
+template <typename S, bool PAIR>
+void* sptr_p(int shape, int bc, bool hd)
 {
     switch (shape) {
-    case PERKS_SHAPE_2D_STAR_R1: return sptr2<PERKS_SHAPE_2D_STAR_R1, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R2: return sptr2<PERKS_SHAPE_2D_STAR_R2, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R3: return sptr2<PERKS_SHAPE_2D_STAR_R3, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R4: return sptr2<PERKS_SHAPE_2D_STAR_R4, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R5: return sptr2<PERKS_SHAPE_2D_STAR_R5, S>(bc, hd);
-    case PERKS_SHAPE_2D_STAR_R6: return sptr2<PERKS_SHAPE_2D_STAR_R6, S>(bc, hd);
-    case PERKS_SHAPE_2D_BOX_R1: return sptr2<PERKS_SHAPE_2D_BOX_R1, S>(bc, hd);
-    case PERKS_SHAPE_2D_BOX_R2: return sptr2<PERKS_SHAPE_2D_BOX_R2, S>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R1: return sptr2<PERKS_SHAPE_2D_STAR_R1, S, PAIR>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R2: return sptr2<PERKS_SHAPE_2D_STAR_R2, S, PAIR>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R3: return sptr2<PERKS_SHAPE_2D_STAR_R3, S, PAIR>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R4: return sptr2<PERKS_SHAPE_2D_STAR_R4, S, PAIR>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R5: return sptr2<PERKS_SHAPE_2D_STAR_R5, S, PAIR>(bc, hd);
+    case PERKS_SHAPE_2D_STAR_R6: return sptr2<PERKS_SHAPE_2D_STAR_R6, S, PAIR>(bc, hd);
+    case PERKS_SHAPE_2D_BOX_R1: return sptr2<PERKS_SHAPE_2D_BOX_R1, S, PAIR>(bc, hd);
+    case PERKS_SHAPE_2D_BOX_R2: return sptr2<PERKS_SHAPE_2D_BOX_R2, S, PAIR>(bc, hd);
     default: return nullptr;
+    }
+}
+
+// pair: the two-columns-per-lane form (FP32, FP16 and BF16 storage only).
+template <typename S>
+void* sptr(int shape, int bc, bool hd, bool pair)
+{
+    if constexpr (std::is_same_v<S, double>) {
+        return pair ? nullptr : sptr_p<S, false>(shape, bc, hd);
+    } else {
+        return pair ? sptr_p<S, true>(shape, bc, hd) : sptr_p<S, false>(shape, bc, hd);
     }
 }
 
