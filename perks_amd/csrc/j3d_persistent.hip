@@ -9,19 +9,23 @@
 // keeps every tile resident on chip for L consecutive time steps:
 //
 //   * The domain is cut into work items: an x-y tile (64 lanes by NW*NY rows,
-//     including a ghost rim of L-1 cells) times a z-slab.
+//     including a ghost rim of L cells in x and L-1 in y) times a z-slab.
 //   * A workgroup marches through its item's slab plane by plane.  Level 0 is
 //     the field as read from HBM; level l (1..L) is the field after l steps.
 //     At each plane, level l computes plane p-l from three planes of level l-1
 //     that it holds in VGPRs (a ring of three plane buffers per level), so a
 //     plane is read from HBM once and written once per L steps.
 //   * East/west neighbours come from the adjacent lane by DPP whole-wave
-//     shifts, north/south from the lane's own rows, and the rows that cross
-//     between the workgroup's waves are exchanged through LDS once per level.
+//     shifts, north/south from the lane's own rows.  The rows that cross
+//     between the workgroup's waves are published to LDS when a level
+//     produces a plane and read by the level above one iteration later, so
+//     the workgroup meets once per plane for all levels.
 //   * Each level's valid region shrinks by one cell per side; the ghost rim
 //     absorbs it, and the last level stores only the item's interior.
 //   * Domain faces apply the boundary rule at every level (clamp: the edge
-//     cell itself, zero: 0), so partial results never leak across a face.
+//     cell itself, zero: 0), so partial results never leak across a face:
+//     lanes beyond an x face hold the boundary value, waves at a y face and
+//     planes at a z face select it.
 //   * After every pass of L steps the grid meets at a barrier (agent-scope
 //     release/acquire, so stores from one XCD's L2 reach the others) and the
 //     two field buffers swap roles.
