@@ -99,7 +99,8 @@ typedef struct perks_stencil_opts {
     /* experimental knobs, 0 = automatic:
      *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 2 = cached stores (default nontemporal)
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
-     *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel).
+     *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel),
+     *           4 = two columns per lane (FP32/FP16/BF16, even widths).
      *           3D other shapes: 0 = default (2), 2 = register streaming, 3 = LDS planes.
      *           2D: 0 = default (4 where it applies, else 2), 1 = LDS tile 128x64 (64x64 FP64),
      *           2 = row streaming through registers, one column per lane, 3 = LDS tile 128x32

@@ -73,6 +73,10 @@ int variant_of(const perks_stencil_opts* o)
     return (v >= 0 && v < kNumVariants) ? v : 0;
 }
 
+// Columns a tile stores with l steps per pass: 64 - 2l, or for the
+// two-column variant 128 minus a rim of l rounded up to even per side.
+int tile_width(const Variant& v, int l) { return v.pair ? 128 - 4 * ((l + 1) / 2) : 64 - 2 * l; }
+
 }  // namespace
 
 // ---- planning --------------------------------------------------------------
@@ -81,9 +85,8 @@ int variant_of(const perks_stencil_opts* o)
 // G workgroups; an item costs its slab height plus the 2*l ghost planes it
 // reads beyond the slab.  Picks the z-slab count that minimises
 // rounds * (planes per item).
-static int best_nzs(int nz, int ny, int nx, int l, int G, int rows)
+static int best_nzs(int nz, int ny, int nx, int l, int G, int rows, int VW)
 {
-    const int VW = 64 - 2 * l;
     const int VH = rows - 2 * (l - 1);
     const long long tiles = (long long)((nx + VW - 1) / VW) * ((ny + VH - 1) / VH);
     int best = 1;
@@ -109,6 +112,7 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     if ((long long)ny * nx * elem_bytes(dtype) >= (long long)kOob) return PERKS_ERR_UNSUPPORTED_SIZE;
     if (occupancy <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
     const Variant v = variant_shape(dtype, variant_of(o));
+    if (v.pair && (nx & 1)) return PERKS_ERR_UNSUPPORTED;  // two columns per lane need an even width
     int occ = std::min(occupancy, std::max(1, kCoopMaxWavesPerCu / v.nw));
     if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
     if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
@@ -122,11 +126,12 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     out->grid[2] = 1;
     out->block = v.nw * 64;
     out->occupancy = occ;
-    out->lds_bytes = (int)(2 * (kMaxLevels - 1) * v.nw * 2 * 64 * (dtype == PERKS_F64 ? 8 : 4));
-    out->tile_x = 64;
+    out->lds_bytes = (int)(2 * (kMaxLevels - 1) * v.nw * 2 * 64 * (dtype == PERKS_F64 || v.pair ? 8 : 4));
+    out->tile_x = v.pair ? 128 : 64;
     out->tile_y = v.nw * v.ny;
     out->reserved[0] = L;
-    for (int l = 1; l <= kMaxLevels; ++l) out->reserved[l] = best_nzs(nz, ny, nx, l, G, v.nw * v.ny);
+    for (int l = 1; l <= kMaxLevels; ++l)
+        out->reserved[l] = best_nzs(nz, ny, nx, l, G, v.nw * v.ny, tile_width(v, l));
     return PERKS_OK;
 }
 

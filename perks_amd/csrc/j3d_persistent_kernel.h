@@ -376,6 +376,390 @@ struct March {
     }
 };
 
+// ---- two columns per lane (FP32 arithmetic) ----------------------------------------
+// Storage access to the cells of an even column x0 and x0 + 1 in one buffer
+// operation, and their rounding to the storage format.
+template <int FMT>
+struct Elem2;
+template <>
+struct Elem2<PERKS_F32> {
+    __device__ static void load(rsrc_t r, uint32_t off, float& a, float& b)
+    {
+        const uint64_t u = (uint64_t)__double_as_longlong(buf_load(r, off, double{}));
+        a = __uint_as_float((uint32_t)u);
+        b = __uint_as_float((uint32_t)(u >> 32));
+    }
+    __device__ static void store(rsrc_t r, uint32_t off, float a, float b)
+    {
+        const uint64_t u = ((uint64_t)__float_as_uint(b) << 32) | __float_as_uint(a);
+        buf_store(r, off, __longlong_as_double((long long)u));
+    }
+    template <bool SR>
+    __device__ static void qr(float&, float&, uint32_t, uint32_t) {}
+};
+template <int FMT>
+struct Elem2x16 {
+    __device__ static void load(rsrc_t r, uint32_t off, float& a, float& b)
+    {
+        const uint32_t w = __float_as_uint(buf_load(r, off, float{}));
+        a = widen_lo<FMT>(w);
+        b = widen_hi<FMT>(w);
+    }
+    // a and b are storage values already: the conversion is exact.
+    __device__ static void store(rsrc_t r, uint32_t off, float a, float b)
+    {
+        buf_store(r, off, __uint_as_float(Store16<FMT>::narrow2(a, b)));
+    }
+    template <bool SR>
+    __device__ static void qr(float& a, float& b, uint32_t idx0, uint32_t key)
+    {
+        if constexpr (SR) {
+            a = Store16<FMT>::widen(narrow_sr<FMT>(a, lowbias32(idx0 ^ key)));
+            b = Store16<FMT>::widen(narrow_sr<FMT>(b, lowbias32((idx0 + 1u) ^ key)));
+        } else {
+            const uint32_t w = Store16<FMT>::narrow2(a, b);
+            a = widen_lo<FMT>(w);
+            b = widen_hi<FMT>(w);
+        }
+    }
+};
+template <>
+struct Elem2<PERKS_F16> : Elem2x16<PERKS_F16> {};
+template <>
+struct Elem2<PERKS_BF16> : Elem2x16<PERKS_BF16> {};
+
+// March with each lane holding two neighbouring columns (a: even x0, b: x0+1),
+// so one load, store, LDS access and pair of lane shifts serves two cells.
+// A tile is 128 columns wide with a ghost rim of G = L rounded up to even
+// columns (stores stay on whole pairs; the grid width must be even).  Level
+// structure, the one-iteration-late row exchange and the face handling are
+// those of March.
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
+struct March2 {
+    using E = Elem2<FMT>;
+    using S = typename Elem<FMT>::S;
+    using T = float;
+    static constexpr int R = NY + 2;
+    static constexpr int D = P - 3;
+    static constexpr int UF = P % 3 == 0 ? P : 3 * P;
+    static constexpr int G = 2 * ((L + 1) / 2);
+    const S* __restrict__ src;
+    S* __restrict__ dst;
+    float2* xch;  // LDS exchange rows [2][kMaxLevels-1][NW][2][64] of pairs
+    int nz, ny, nx;
+    size_t plane;
+    uint32_t plane_bytes;
+    int lane, wave;
+    T k[8];
+    uint32_t roff[R];
+    uint32_t soff[NY];
+    bool xlo, xhi;  // this pair holds column 0 (half a) / nx-1 (half b)
+    bool ylo[NY], yhi[NY];
+    bool edge;
+    int x0;
+    bool xedge, pout;
+    int lane_w, lane_e;
+    int lo[L + 1], hi[L + 1];
+    int base;
+    uint32_t cbase, plane32;
+    uint32_t key[L + 1];
+
+    T w0[P][R][2];
+    T wl[L > 1 ? L - 1 : 1][3][NY][2];
+
+    __device__ __forceinline__ void load0(int q, int slot)
+    {
+        const int qc = q < nz ? q : nz - 1;
+        const rsrc_t rs = make_rsrc(src + (size_t)qc * plane, plane_bytes);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            T a, b;
+            E::load(rs, roff[r], a, b);
+            if constexpr (BC == PERKS_BC_CLAMP) {
+                // A pair beyond a side face loads the edge pair: both halves take the edge column.
+                w0[slot][r][0] = x0 > nx - 2 ? b : a;
+                w0[slot][r][1] = x0 < 0 ? a : b;
+            } else {
+                w0[slot][r][0] = a;
+                w0[slot][r][1] = b;
+            }
+        }
+    }
+
+    template <bool EDGE, int RB>
+    __device__ __forceinline__ void compute(const T (&b)[RB][2], const T (&c)[RB][2], const T (&u)[RB][2], int o,
+                                            const T (&c_lo)[2], const T (&c_hi)[2], int p, int lv,
+                                            T (&out)[NY][2]) const
+    {
+        const bool zlo = EDGE && p == 0;
+        const bool zhi = EDGE && p == nz - 1;
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            const T ca = c[o + r][0], cb = c[o + r][1];
+            // column x0-1 is half b of the lane before; column x0+2 half a of the lane after
+            T wa = dpp_shift0<kDppWaveShr1>(cb), ea = cb;
+            T wb = ca, eb = dpp_shift0<kDppWaveShl1>(ca);
+            T na = r + 1 < NY ? c[o + r + 1][0] : c_hi[0], nb = r + 1 < NY ? c[o + r + 1][1] : c_hi[1];
+            T sa = r > 0 ? c[o + r - 1][0] : c_lo[0], sb = r > 0 ? c[o + r - 1][1] : c_lo[1];
+            T ta = u[o + r][0], tb = u[o + r][1], ba = b[o + r][0], bb = b[o + r][1];
+            if (EDGE) {
+                const T bva = bc_value<T, BC>(ca), bvb = bc_value<T, BC>(cb);
+                wa = xlo ? bva : wa;
+                eb = xhi ? bvb : eb;
+                na = yhi[r] ? bva : na;
+                nb = yhi[r] ? bvb : nb;
+                sa = ylo[r] ? bva : sa;
+                sb = ylo[r] ? bvb : sb;
+                ta = zhi ? bva : ta;
+                tb = zhi ? bvb : tb;
+                ba = zlo ? bva : ba;
+                bb = zlo ? bvb : bb;
+            }
+            T va = star1_update<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca);
+            T vb = star1_update<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb);
+            E::template qr<SR>(va, vb, (uint32_t)p * plane32 + cbase + (uint32_t)(r * nx), key[lv]);
+            out[r][0] = va;
+            out[r][1] = vb;
+        }
+    }
+
+    template <int RB>
+    __device__ __forceinline__ void level(const T (&b)[RB][2], const T (&c)[RB][2], const T (&u)[RB][2], int o,
+                                          const T (&c_lo)[2], const T (&c_hi)[2], int p, int lv,
+                                          T (&out)[NY][2]) const
+    {
+        if (edge || p == 0 || p == nz - 1) compute<true>(b, c, u, o, c_lo, c_hi, p, lv, out);
+        else compute<false>(b, c, u, o, c_lo, c_hi, p, lv, out);
+    }
+
+    __device__ __forceinline__ void side_faces(T (&out)[NY][2]) const
+    {
+        if (!xedge) return;
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            if constexpr (BC == PERKS_BC_ZERO) {
+                out[r][0] = pout ? (T)0 : out[r][0];
+                out[r][1] = pout ? (T)0 : out[r][1];
+            } else {
+                const T vw = read_lane_t<T>(out[r][0], lane_w);
+                const T ve = read_lane_t<T>(out[r][1], lane_e);
+                out[r][0] = x0 < 0 ? vw : (x0 > nx - 2 ? ve : out[r][0]);
+                out[r][1] = x0 < 0 ? vw : (x0 > nx - 2 ? ve : out[r][1]);
+            }
+        }
+    }
+
+    __device__ __forceinline__ float2* xrows(int par, int lv) const
+    {
+        return xch + ((size_t)(par * (kMaxLevels - 1) + lv - 1) * NW) * 2 * 64;
+    }
+
+    __device__ __forceinline__ void publish(int lv, int par, const T (&out)[NY][2], T (&ring)[NY][2]) const
+    {
+        float2* x = xrows(par, lv);
+        x[(wave * 2 + 0) * 64 + lane] = make_float2(out[0][0], out[0][1]);
+        x[(wave * 2 + 1) * 64 + lane] = make_float2(out[NY - 1][0], out[NY - 1][1]);
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            ring[r][0] = out[r][0];
+            ring[r][1] = out[r][1];
+        }
+    }
+
+    __device__ __forceinline__ void fetch(int lv, int par, T (&c_lo)[2], T (&c_hi)[2]) const
+    {
+        const float2* x = xrows(par, lv);
+        const int wb = wave > 0 ? wave - 1 : 0;
+        const int wa = wave < NW - 1 ? wave + 1 : NW - 1;
+        const float2 lo2 = x[(wb * 2 + 1) * 64 + lane];
+        const float2 hi2 = x[(wa * 2 + 0) * 64 + lane];
+        c_lo[0] = lo2.x;
+        c_lo[1] = lo2.y;
+        c_hi[0] = hi2.x;
+        c_hi[1] = hi2.y;
+    }
+
+    __device__ __forceinline__ void store(int p, const T (&out)[NY][2]) const
+    {
+        const rsrc_t rs = make_rsrc(dst + (size_t)p * plane, plane_bytes);
+#pragma unroll
+        for (int r = 0; r < NY; ++r) E::store(rs, soff[r], out[r][0], out[r][1]);
+    }
+
+    static constexpr int md(int a) { return ((a % 3) + 3) % 3; }
+    static constexpr int mp(int a) { return ((a % P) + P) % P; }
+
+    template <int U, int LV>
+    __device__ __forceinline__ void upper(int i, const T (&hl)[L > 1 ? L - 1 : 1][2][2])
+    {
+        if constexpr (LV <= L) {
+            const int p = i - LV;
+            if (p >= lo[LV] && p < hi[LV]) {
+                T out[NY][2];
+                level(wl[LV - 2][md(U - LV - 1)], wl[LV - 2][md(U - LV)], wl[LV - 2][md(U - LV + 1)], 0, hl[LV - 2][0],
+                      hl[LV - 2][1], p, LV, out);
+                if constexpr (LV == L) {
+                    store(p, out);
+                } else {
+                    side_faces(out);
+                    publish(LV, i & 1, out, wl[LV - 1][md(U - LV)]);
+                }
+            }
+            upper<U, LV + 1>(i, hl);
+        }
+    }
+
+    template <int U>
+    __device__ __forceinline__ void stage(int i)
+    {
+        load0(i + D, mp(U + D));
+        T hl[L > 1 ? L - 1 : 1][2][2];
+#pragma unroll
+        for (int lv = 2; lv <= L; ++lv) fetch(lv - 1, (i - 1) & 1, hl[lv - 2][0], hl[lv - 2][1]);
+        const int p = i - 1;
+        if (p >= lo[1] && p < hi[1]) {
+            T out[NY][2];
+            const T(&c)[R][2] = w0[mp(U - 1)];
+            level(w0[mp(U - 2)], c, w0[mp(U)], 1, c[0], c[NY + 1], p, 1, out);
+            if constexpr (L == 1) {
+                store(p, out);
+            } else {
+                side_faces(out);
+                publish(1, i & 1, out, wl[0][md(U - 1)]);
+            }
+        }
+        upper<U, 2>(i, hl);
+        if constexpr (L > 1) lds_barrier();
+    }
+
+    template <int U>
+    __device__ __forceinline__ bool trip(int i, int iend)
+    {
+        if constexpr (U < UF) {
+            if (i > iend) return false;
+            stage<(U + 1) % UF>(i);
+            return trip<U + 1>(i + 1, iend);
+        }
+        return true;
+    }
+
+    __device__ void run(int item, int nzs)
+    {
+        lds_barrier();
+        constexpr int VW = 128 - 2 * G;
+        const int VH = NW * NY - 2 * (L - 1);
+        const int ntx = (nx + VW - 1) / VW;
+        const int nty = (ny + VH - 1) / VH;
+        const int tx = item % ntx;
+        const int ty = (item / ntx) % nty;
+        const int zsl = item / (ntx * nty);
+        const int zs = (int)(((long long)zsl * nz) / nzs);
+        const int ze = (int)(((long long)(zsl + 1) * nz) / nzs);
+
+        const int vx0 = tx * VW, vx1 = min(vx0 + VW, nx);
+        const int vy0 = ty * VH, vy1 = min(vy0 + VH, ny);
+        const int xw0 = vx0 - G;  // column of lane 0's half a (even)
+        x0 = xw0 + 2 * lane;
+        const int xc = x0 < 0 ? 0 : (x0 > nx - 2 ? nx - 2 : x0);
+        const int y0 = vy0 - (L - 1) + wave * NY;
+        xlo = x0 == 0;
+        xhi = x0 + 1 == nx - 1;
+        pout = x0 < 0 || x0 > nx - 2;
+        cbase = (uint32_t)y0 * (uint32_t)nx + (uint32_t)x0;
+        const bool xstore = x0 >= vx0 && x0 < vx1;
+        xedge = xw0 < 0 || xw0 + 127 > nx - 1;
+        lane_w = min(max(-xw0 / 2, 0), 63);
+        lane_e = min(max((nx - 2 - xw0) / 2, 0), 63);
+        edge = y0 <= 0 || y0 + NY - 1 >= ny - 1;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int yy = y0 + r - 1;
+            const int yc = yy < 0 ? 0 : (yy >= ny ? ny - 1 : yy);
+            const bool in = yy >= 0 && yy < ny && !pout;
+            roff[r] = (BC == PERKS_BC_ZERO && !in) ? kOob : (uint32_t)((yc * nx + xc) * (int)sizeof(S));
+        }
+#pragma unroll
+        for (int r = 0; r < NY; ++r) {
+            const int y = y0 + r;
+            ylo[r] = y == 0;
+            yhi[r] = y == ny - 1;
+            soff[r] = (xstore && y >= vy0 && y < vy1) ? (uint32_t)((y * nx + x0) * (int)sizeof(S)) : kOob;
+        }
+#pragma unroll
+        for (int l = 0; l <= L; ++l) {
+            lo[l] = max(0, zs - (L - l));
+            hi[l] = min(nz, ze + (L - l));
+        }
+        base = lo[0];
+#pragma unroll
+        for (int q = 0; q <= D; ++q) load0(base + q, q);
+        const int iend = hi[L] - 1 + L;
+        for (int i = base + 1; trip<0>(i, iend); i += UF) {
+        }
+    }
+};
+
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
+__device__ __forceinline__ void run_pass2(const typename Elem<FMT>::S* src, typename Elem<FMT>::S* dst, float2* xch,
+                                          const PersistArgs& a, const float* k, int lane, int wave, int step0)
+{
+    constexpr int VW = 128 - 2 * (2 * ((L + 1) / 2));
+    const int VH = NW * NY - 2 * (L - 1);
+    const int nitems = ((a.nx + VW - 1) / VW) * ((a.ny + VH - 1) / VH) * a.nzs[L];
+    const int Gd = gridDim.x;
+    const int me = xcd_tile_id(blockIdx.x, Gd);
+    for (int item = me; item < nitems; item += Gd) {
+        March2<FMT, BC, NY, NW, P, L, HAS_DIV, SR> m;
+        m.src = src;
+        m.dst = dst;
+        m.xch = xch;
+        m.nz = a.nz;
+        m.ny = a.ny;
+        m.nx = a.nx;
+        m.plane = (size_t)a.ny * a.nx;
+        m.plane_bytes = (uint32_t)(m.plane * sizeof(typename Elem<FMT>::S));
+        m.lane = lane;
+        m.wave = wave;
+        m.plane32 = (uint32_t)m.plane;
+        if constexpr (SR) {
+#pragma unroll
+            for (int l = 1; l <= L; ++l) m.key[l] = sr_key((uint32_t)(step0 + l - 1), a.seed);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m.k[i] = k[i];
+        m.run(item, a.nzs[L]);
+    }
+}
+
+template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR>
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE)))
+j3d_star1_persistent2_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S* buf_b, PersistArgs a, KCoefs cf)
+{
+    using S = typename Elem<FMT>::S;
+    __shared__ float2 xch[2 * (kMaxLevels - 1) * NW * 2 * 64];
+    float k[8];
+    load_coefs7<float>(cf, k);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int L = a.levels;
+    const int npass = (a.steps + L - 1) / L;
+    S* src = buf_a;
+    S* dst = buf_b;
+    for (int pass = 0; pass < npass; ++pass) {
+        const int lv = (pass == npass - 1) ? a.steps - pass * L : L;
+        switch (lv) {
+        case 1: run_pass2<FMT, BC, NY, NW, P, 1, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 2: run_pass2<FMT, BC, NY, NW, P, 2, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 3: run_pass2<FMT, BC, NY, NW, P, 3, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        default: run_pass2<FMT, BC, NY, NW, P, 4, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        }
+        S* t = src;
+        src = dst;
+        dst = t;
+        if (pass + 1 < npass && !grid_sync(a.sync, gridDim.x, a.timeout)) return;
+    }
+}
+
 template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
 __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typename Elem<FMT>::S* dst,
                                          typename Elem<FMT>::T* xch, const PersistArgs& a,
@@ -449,13 +833,18 @@ j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S*
 //   1  8 waves x 4 rows, 9 slots                  as 0 with 9 slots
 //   2  8 waves x 4 rows (two workgroups per CU)   4 waves x 4 rows (two per CU)
 //   3  16 waves x 2 rows                          16 waves x 2 rows
+//   4  8 waves x 4 rows, two columns per lane     (as 0)
+//      (a 128 x 32 tile; even grid widths; at 512^3 5% slower than 0 for
+//      FP32 and 2% for BF16: twice the registers halve the occupancy)
 // Selected through opts->tune[4].
 struct Variant {
     int ny, nw, p, wpe;
+    int pair;  // two columns per lane (March2)
 };
-constexpr Variant kVarF32[] = {{4, 16, 6, 1}, {4, 8, 9, 1}, {4, 8, 6, 1}, {2, 16, 6, 1}};
-constexpr Variant kVarF64[] = {{4, 8, 6, 1}, {4, 8, 9, 1}, {4, 4, 6, 1}, {2, 16, 6, 1}};
-constexpr int kNumVariants = 4;
+constexpr Variant kVarF32[] = {{4, 16, 6, 1, 0}, {4, 8, 9, 1, 0}, {4, 8, 6, 1, 0}, {2, 16, 6, 1, 0}, {4, 8, 6, 1, 1}};
+constexpr Variant kVarF64[] = {{4, 8, 6, 1, 0}, {4, 8, 9, 1, 0}, {4, 4, 6, 1, 0}, {2, 16, 6, 1, 0}, {4, 8, 6, 1, 0}};
+constexpr int kNumVariants = 5;
+constexpr int kPairVariant = 4;
 __host__ __device__ constexpr Variant variant_shape(int fmt, int v)
 {
     return fmt == PERKS_F64 ? kVarF64[v] : kVarF32[v];
@@ -465,7 +854,10 @@ template <int FMT, int BC, bool HAS_DIV, int V, bool SR>
 void* kernel_ptr()
 {
     constexpr Variant v = variant_shape(FMT, V);
-    return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR>);
+    if constexpr (v.pair != 0 && FMT != PERKS_F64)
+        return reinterpret_cast<void*>(&j3d_star1_persistent2_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR>);
+    else
+        return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR>);
 }
 
 template <int FMT, int BC, bool HAS_DIV, bool SR>
@@ -475,6 +867,7 @@ void* pick_variant(int var)
     case 1: return kernel_ptr<FMT, BC, HAS_DIV, 1, SR>();
     case 2: return kernel_ptr<FMT, BC, HAS_DIV, 2, SR>();
     case 3: return kernel_ptr<FMT, BC, HAS_DIV, 3, SR>();
+    case 4: return kernel_ptr<FMT, BC, HAS_DIV, 4, SR>();
     default: return kernel_ptr<FMT, BC, HAS_DIV, 0, SR>();
     }
 }

@@ -49,7 +49,7 @@ def test_3d7pt_512_persistent_variants_equal_per_step(gpu, dtype, boundary):
     init = torch.empty(n ** 3, dtype=tdt, device="cuda")
     P.fill_uniform(init, n ** 3, 1, 0.0, 1.0, dtype=dtype)
     ref, _ = run3(torch, init, n, steps, dtype, method="plain", boundary=boundary)
-    for var in range(4):
+    for var in range(5):
         got, rep = run3(torch, init, n, steps, dtype, method="persistent", boundary=boundary,
                         tune=[0, 0, 4, 0, var])
         assert rep.reserved[0] == 4
