@@ -28,6 +28,10 @@ Other workloads (--workload; each prints its own JSON line):
                against solving them one after another with the per-step kernel.
   cg           CG on the 7-point Poisson matrix, 256^3, CSR and DIA, FP64 and
                FP32-inner/FP64-refinement, to ||r||/||b|| < 1e-10.
+  j3d-tune     the configuration table of the 3D path at 512^3 (VGPRs, spills,
+               LDS, occupancy of every method / tile variant / pass depth) and
+               the autotuner's timing of each, for the 7-point star in every
+               storage format and the 27-point box in FP64 / FP32.
 """
 from __future__ import annotations
 
@@ -59,7 +63,8 @@ def parse():
     ap.add_argument("--no-per-step", action="store_true", help="skip the per-step comparison")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
-    ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j3d-shapes", "j2d-storage", "batch2d", "cg"],
+    ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j3d-shapes", "j2d-storage", "batch2d", "cg",
+                                                  "j3d-tune"],
                     default="j3d")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
@@ -345,6 +350,45 @@ def run_j3d_shapes(args):
     print(json.dumps(out), flush=True)
 
 
+def run_j3d_tune(args):
+    import torch
+    import perks_amd as P
+
+    n = args.size
+    cells = n ** 3
+    sample = 12
+    tables = {}
+    best = {}
+    for shape, dtypes in (("3d7pt", ("f64", "f32", "f16", "bf16")), ("3d27pt", ("f64", "f32"))):
+        for dt in dtypes:
+            tdt = {"f64": torch.float64, "f32": torch.float32}.get(dt, torch.int16)
+            init = torch.empty(cells, dtype=tdt, device="cuda")
+            P.fill_uniform(init, cells, args.seed, 0.0, 1.0, dtype=dt)
+            opts, rows = P.j3d_autotune(init, n, n, n, dtype=dt, sample_steps=sample, shape=shape, boundary="zero")
+            torch.cuda.synchronize()
+            key = f"{shape}_{dt}"
+            tables[key] = rows
+            ok = [r for r in rows if r["status"] == 0]
+            b = max(ok, key=lambda r: r["gcells_per_s"])
+            best[key] = {"method": b["method"], "variant": b["variant"], "levels": b["levels"],
+                         "gcells_per_s": round(b["gcells_per_s"], 2), "tune2": opts.tune[2], "tune4": opts.tune[4]}
+            for r in rows:
+                r["gcells_per_s"] = round(r["gcells_per_s"], 2)
+            del init
+            torch.cuda.empty_cache()
+    v = best["3d7pt_f64"]
+    out = {
+        "metric": "autotuned 3D configuration at 512^3 (sample of 12 steps per configuration)",
+        "value": v["gcells_per_s"], "unit": "GCells/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(sample * cells / (v["gcells_per_s"] * 1e9) * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Jacobi {n}^3, zero boundary, reference weights, input uniform[0,1) (seed "
+                               f"{args.seed}); every configuration timed on {sample} steps; value = best 3d7pt FP64",
+                   "size": n, "sample_steps": sample, "best": best, "tables": tables},
+    }
+    print(json.dumps(out), flush=True)
+
+
 # ---------------------------------------------------------------------------- 2D storage precision
 def _errors(torch, got, ref):
     """max and RMS of |got - ref|, absolute and relative to max|ref| / RMS(ref)."""
@@ -622,7 +666,7 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     {"j3d": run_j3d, "j3d-storage": run_j3d_storage, "j3d-shapes": run_j3d_shapes, "j2d-storage": run_j2d_storage,
-     "batch2d": run_batch2d, "cg": run_cg}[args.workload](args)
+     "batch2d": run_batch2d, "cg": run_cg, "j3d-tune": run_j3d_tune}[args.workload](args)
 
 
 if __name__ == "__main__":

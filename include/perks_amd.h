@@ -99,6 +99,8 @@ typedef struct perks_stencil_opts {
     /* experimental knobs, 0 = automatic:
      *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 2 = cached stores (default nontemporal)
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
+     *           (3D 7-point default: 4; 3 for FP16 and for BF16 with nearest rounding;
+     *           perks_j3d_autotune measures every choice)
      *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel),
      *           4 = two columns per lane (FP32/FP16/BF16, even widths).
      *           3D other shapes: 0 = default (2), 2 = register streaming, 3 = LDS planes.
@@ -204,6 +206,46 @@ int perks_j3d_device_f32(float* d_a, float* d_b, int height, int width_y, int wi
  * counterpart: the reduced-precision form is an addition of this library. */
 int perks_j3d_device(int dtype, void* d_a, void* d_b, int height, int width_y, int width_x, int steps,
                      const perks_stencil_opts* opts, void* stream, perks_run_report* report);
+
+/* ---- 3D configuration table and autotuner ---------------------------------- */
+/* One launch configuration of the 3D path for a problem (method, tile
+ * variant, time steps per pass) and what its loaded code object reports.
+ * Replaces one row of the reference's register-folder table
+ * (stencil/3dstencil/perksconfig.cuh, regfolder<>), which the reference
+ * derives at build time from nvcc's register and spill report
+ * (ProcessCompilerLog.py:47-75, ProcessRst2cuh.py:5-51). */
+typedef struct perks_kernel_config {
+    int32_t method;        /* PERKS_METHOD_PLAIN or PERKS_METHOD_PERSISTENT       */
+    int32_t variant;       /* opts.tune[4]                                        */
+    int32_t levels;        /* opts.tune[2]: time steps per pass (1 for plain)     */
+    int32_t block;         /* threads per workgroup                               */
+    int32_t vgprs;         /* architected VGPRs per lane                          */
+    int32_t spill_bytes;   /* scratch bytes per lane; nonzero means spills        */
+    int32_t lds_bytes;     /* LDS per workgroup                                   */
+    int32_t occupancy;     /* workgroups per CU the launch would use              */
+    int32_t tile_x;        /* x-y tile of one work item                           */
+    int32_t tile_y;
+    int32_t status;        /* PERKS_OK, or why the configuration does not apply   */
+    int32_t reserved0;
+    double gcells_per_s;   /* measured by perks_j3d_autotune (0 from the table)   */
+    int32_t reserved[4];
+} perks_kernel_config;
+
+/* Lists every configuration the library holds for this problem (dtype,
+ * sizes, opts' shape, boundary, coefficients and rounding), writing at most
+ * `capacity` entries and the total to *count.  Needs the device (attribute
+ * and occupancy queries) but launches nothing. */
+int perks_j3d_kernel_table(int dtype, int height, int width_y, int width_x, const perks_stencil_opts* opts,
+                           perks_kernel_config* table, int capacity, int* count);
+/* Times every configuration of the table whose status is PERKS_OK on
+ * `sample_steps` steps (<= 0: 12, a multiple of every pass depth) of the
+ * field d_in (left unchanged; the runs use two scratch fields the call
+ * allocates and frees), then writes the fastest into opts: method, tune[2]
+ * and tune[4].  Every configuration computes the same bits, so the choice
+ * changes only the speed.  table/count as above (table may be NULL). */
+int perks_j3d_autotune(int dtype, const void* d_in, int height, int width_y, int width_x, int sample_steps,
+                       perks_stencil_opts* opts, perks_kernel_config* table, int capacity, int* count,
+                       void* stream);
 
 /* ---- 2D Jacobi ------------------------------------------------------------ */
 /* Host buffers h_in[width_y][width_x] (x fastest), as the reference's

@@ -81,6 +81,15 @@ class PlanInfo(C.Structure):
     ]
 
 
+class KernelConfig(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32), ("variant", C.c_int32), ("levels", C.c_int32), ("block", C.c_int32),
+        ("vgprs", C.c_int32), ("spill_bytes", C.c_int32), ("lds_bytes", C.c_int32), ("occupancy", C.c_int32),
+        ("tile_x", C.c_int32), ("tile_y", C.c_int32), ("status", C.c_int32), ("reserved0", C.c_int32),
+        ("gcells_per_s", C.c_double), ("reserved", C.c_int32 * 4),
+    ]
+
+
 class Grid2d(C.Structure):
     _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("width_y", C.c_int32), ("width_x", C.c_int32),
                 ("steps", C.c_int32), ("result_in_b", C.c_int32)]
@@ -144,6 +153,10 @@ SIGNATURES = {
     "perks_j3d_device_f64": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
     "perks_j3d_device_f32": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
     "perks_j3d_device": (_I, [_I, _P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),
+    "perks_j3d_kernel_table": (_I, [_I, _I, _I, _I, C.POINTER(StencilOpts), C.POINTER(KernelConfig), _I,
+                                    C.POINTER(C.c_int)]),
+    "perks_j3d_autotune": (_I, [_I, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), C.POINTER(KernelConfig), _I,
+                                C.POINTER(C.c_int), _P]),
     "perks_fill_uniform_f64": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
     "perks_fill_uniform_f32": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),
     "perks_fill_uniform_f16": (_I, [_P, C.c_size_t, C.c_uint64, _D, _D, _P]),

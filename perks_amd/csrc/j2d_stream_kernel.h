@@ -5,8 +5,10 @@
 // A work item is a strip of 64 columns (one per lane) and a segment of rows
 // of one grid; each wave runs its own items, so there is no LDS and no
 // workgroup barrier.  The wave marches down the segment one row per
-// iteration: level 0 is the field row read from HBM (prefetched D rows
-// ahead), and level l (1..L) computes row i - R*l from the 2R+1 rows of level
+// iteration: level 0 is the field row read from HBM (requested D rows ahead
+// and kept as loaded until it enters the window, where it is widened, so
+// nothing waits on a load before its first use), and level l (1..L) computes
+// row i - R*l from the 2R+1 rows of level
 // l-1 it keeps in a register ring.  East/west neighbours come from the
 // adjacent lanes by DPP whole-wave shifts; box shapes shift every row of
 // their window once when it is produced, star shapes shift only the centre
@@ -16,8 +18,8 @@
 //
 // Boundaries: at every level, the cells just outside the grid hold what the
 // boundary rule reads there -- 0 (zero) or the nearest edge cell (clamp) --
-// so the update itself never tests for a face.  Level 0 gets them from the
-// load (clamped indices, or 0); higher levels copy the edge row into the rows
+// so the update itself never tests for a face.  Level 0 gets them from
+// clamped loads (zeroed as the row enters the window); higher levels copy the edge row into the rows
 // beyond the top and bottom faces when the edge row is produced, and the
 // edge column into the lanes beyond the side faces (readlane).  This is the
 // per-step kernel's rule applied level by level, so a pass equals L stored
@@ -72,8 +74,7 @@ struct Strip {
     static constexpr bool BOX = SD.is_box != 0;
     static constexpr int Q = 2 * R + 1;            // rows of a level's window
     static constexpr int D = Q * ((8 + Q - 1) / Q);  // level-0 rows loaded ahead (at least 8, a multiple of Q)
-    static constexpr int P = Q + D;                // level-0 ring slots
-    static constexpr int UF = P;         // stages per loop trip (a multiple of Q: ring slots are static)
+    static constexpr int UF = D;  // stages per loop trip: a multiple of Q and D, so every ring slot is static
     static constexpr int NS = BOX ? Q : 1;
 
     const S* src;
@@ -92,16 +93,23 @@ struct Strip {
     bool sr;
     uint32_t key[L + 1];
 
-    C raw[P];         // level-0 rows as loaded
-    C win[L][Q][NS];  // level l's window (level 0 only for box shapes); boxes keep the x shifts
+    using Raw = typename Fmt<S>::Raw;
+    Raw fl[D];        // level-0 rows in flight, as loaded (row i in slot (i - i0) mod D)
+    C win[L][Q][NS];  // level l's window; boxes keep the x shifts
 
-    static constexpr int mp(int a) { return ((a % P) + P) % P; }
     static constexpr int mq(int a) { return ((a % Q) + Q) % Q; }
 
-    __device__ __forceinline__ C load(int q) const
+    __device__ __forceinline__ Raw load(int q) const
     {
         const int qc = q < 0 ? 0 : (q >= ny ? ny - 1 : q);
-        const C v = Fmt<S>::ld(src + (size_t)qc * nx, (size_t)xc);
+        return Fmt<S>::ld_raw(src + (size_t)qc * nx, (size_t)xc);
+    }
+
+    // Row q of level 0 from its loaded word, at its first use (D rows after
+    // its load was issued): widened, and 0 outside the grid (zero boundary).
+    __device__ __forceinline__ C arrive(int q, Raw w) const
+    {
+        const C v = Fmt<S>::widen(w);
         if constexpr (BC == PERKS_BC_ZERO) return (q < 0 || q >= ny || xout) ? (C)0 : v;
         return v;
     }
@@ -113,8 +121,7 @@ struct Strip {
         if constexpr (BOX) return win[LV][mq(rel)][R + dx];
         else {
             if (rel == centre_rel) return csh[R + dx];
-            if constexpr (LV == 0) return raw[mp(rel)];
-            else return win[LV][mq(rel)][0];
+            return win[LV][mq(rel)][0];
         }
     }
 
@@ -147,10 +154,7 @@ struct Strip {
             if (q >= lo[LV1] && q < hi[LV1] && q < ny) {
                 C csh[Q];
                 if constexpr (!BOX) {
-                    C c;
-                    if constexpr (LV == 0) c = raw[mp(REL)];
-                    else c = win[LV][mq(REL)][0];
-                    shifts<R>(c, csh);
+                    shifts<R>(win[LV][mq(REL)][0], csh);
                 }
                 v = eval_shape<SHAPE, C, HAS_DIV>(k, kdiv, kd, [&](int dy, int dx) -> C {
                     return cell<LV>(REL + dy, dx, csh, REL);
@@ -187,17 +191,24 @@ struct Strip {
         }
     }
 
-    // Iteration i (stage U = (i - i0) mod UF): level-0 row i has arrived; row
-    // i + D is requested into the slot row i - 2R - 1 held.
+    // Iteration i (stage U = (i - i0) mod UF): level-0 row i, loaded D
+    // iterations ago, enters the window, and row i + D is requested into its
+    // in-flight slot.
     template <int U>
     __device__ __forceinline__ void stage(int i)
     {
-        raw[mp(U + D)] = load(i + D);
+        // The next load is issued before row i is waited for, so D loads
+        // stay in flight across the wait.
+        const Raw w = fl[U];
+        fl[U] = load(i + D);
+        const C c = arrive(i, w);
         if constexpr (BOX) {
             C sh[Q];
-            shifts<R>(raw[mp(U)], sh);
+            shifts<R>(c, sh);
 #pragma unroll
             for (int t = 0; t < Q; ++t) win[0][mq(U)][t] = sh[t];
+        } else {
+            win[0][mq(U)][0] = c;
         }
         level<U, 0>(i);
     }
@@ -232,7 +243,7 @@ struct Strip {
         const int i0 = ys - R * L;
         const int iend = ye - 1 + R * L;
 #pragma unroll
-        for (int t = 0; t < D; ++t) raw[t] = load(i0 + t);
+        for (int t = 0; t < D; ++t) fl[t] = load(i0 + t);
         for (int i = i0; trip<0>(i, iend); i += UF) {
         }
     }
@@ -245,6 +256,13 @@ template <typename S>
 struct Pair2;
 template <>
 struct Pair2<float> {
+    using Raw = float2;
+    __device__ static Raw ld_raw(const float* row, size_t x0) { return *reinterpret_cast<const float2*>(row + x0); }
+    __device__ static void unpack(Raw w, float& a, float& b)
+    {
+        a = w.x;
+        b = w.y;
+    }
     __device__ static void ld(const float* row, size_t x0, float& a, float& b)
     {
         const float2 v = *reinterpret_cast<const float2*>(row + x0);
@@ -259,6 +277,13 @@ struct Pair2<float> {
 };
 template <typename S16, int FMT>
 struct Pair16 {
+    using Raw = uint32_t;
+    __device__ static Raw ld_raw(const S16* row, size_t x0) { return *reinterpret_cast<const uint32_t*>(row + x0); }
+    __device__ static void unpack(Raw w, float& a, float& b)
+    {
+        a = widen_lo<FMT>(w);
+        b = widen_hi<FMT>(w);
+    }
     __device__ static void ld(const S16* row, size_t x0, float& a, float& b)
     {
         const uint32_t w = *reinterpret_cast<const uint32_t*>(row + x0);
@@ -327,8 +352,7 @@ struct Strip2 {
     static constexpr bool BOX = SD.is_box != 0;
     static constexpr int Q = 2 * R + 1;
     static constexpr int D = Q * ((8 + Q - 1) / Q);
-    static constexpr int P = Q + D;
-    static constexpr int UF = P;
+    static constexpr int UF = D;
     static constexpr int NS = BOX ? Q : 1;
 
     const S* src;
@@ -346,25 +370,33 @@ struct Strip2 {
     bool sr;
     uint32_t key[L + 1];
 
-    C raw[P][2];
+    using Raw = typename Pair2<S>::Raw;
+    Raw fl[D];  // level-0 rows in flight, as loaded (row i in slot (i - i0) mod D)
     C win[L][Q][NS][2];
 
-    static constexpr int mp(int v) { return ((v % P) + P) % P; }
     static constexpr int mq(int v) { return ((v % Q) + Q) % Q; }
 
-    __device__ __forceinline__ void load(int q, C (&out)[2]) const
+    __device__ __forceinline__ Raw load(int q) const
     {
         const int qc = q < 0 ? 0 : (q >= ny ? ny - 1 : q);
         const int xc = x0 < 0 ? 0 : (x0 > nx - 2 ? nx - 2 : x0);
+        return Pair2<S>::ld_raw(src + (size_t)qc * nx, (size_t)xc);
+    }
+
+    // Row q of level 0 from its loaded word, at its first use (D rows after
+    // its load was issued): unpacked, with the boundary rule applied to pairs
+    // outside the grid.
+    __device__ __forceinline__ void arrive(int q, Raw w, C& oa, C& ob) const
+    {
         C a, b;
-        Pair2<S>::ld(src + (size_t)qc * nx, (size_t)xc, a, b);
+        Pair2<S>::unpack(w, a, b);
         if constexpr (BC == PERKS_BC_ZERO) {
             const bool z = q < 0 || q >= ny || pout;
-            out[0] = z ? (C)0 : a;
-            out[1] = z ? (C)0 : b;
+            oa = z ? (C)0 : a;
+            ob = z ? (C)0 : b;
         } else {
-            out[0] = x0 > nx - 2 ? b : a;  // right of the grid: column nx-1 twice
-            out[1] = x0 < 0 ? a : b;       // left of the grid: column 0 twice
+            oa = x0 > nx - 2 ? b : a;  // right of the grid: column nx-1 twice
+            ob = x0 < 0 ? a : b;       // left of the grid: column 0 twice
         }
     }
 
@@ -374,8 +406,7 @@ struct Strip2 {
         if constexpr (BOX) return win[LV][mq(rel)][R + dx][h];
         else {
             if (rel == centre) return h ? csb[R + dx] : csa[R + dx];
-            if constexpr (LV == 0) return raw[mp(rel)][h];
-            else return win[LV][mq(rel)][0][h];
+            return win[LV][mq(rel)][0][h];
         }
     }
 
@@ -404,15 +435,7 @@ struct Strip2 {
             if (q >= lo[LV1] && q < hi[LV1] && q < ny) {
                 C csa[Q], csb[Q];
                 if constexpr (!BOX) {
-                    C ca, cb;
-                    if constexpr (LV == 0) {
-                        ca = raw[mp(REL)][0];
-                        cb = raw[mp(REL)][1];
-                    } else {
-                        ca = win[LV][mq(REL)][0][0];
-                        cb = win[LV][mq(REL)][0][1];
-                    }
-                    pair_shifts<R>(ca, cb, csa, csb);
+                    pair_shifts<R>(win[LV][mq(REL)][0][0], win[LV][mq(REL)][0][1], csa, csb);
                 }
                 va = eval_shape<SHAPE, C, HAS_DIV>(k, kdiv, kd, [&](int dy, int dx) -> C {
                     return cell<LV>(REL + dy, dx, 0, csa, csb, REL);
@@ -470,15 +493,21 @@ struct Strip2 {
     template <int U>
     __device__ __forceinline__ void stage(int i)
     {
-        load(i + D, raw[mp(U + D)]);
+        const Raw w = fl[U];
+        fl[U] = load(i + D);  // issued before row i is waited for
+        C a, b;
+        arrive(i, w, a, b);
         if constexpr (BOX) {
             C sa[Q], sb[Q];
-            pair_shifts<R>(raw[mp(U)][0], raw[mp(U)][1], sa, sb);
+            pair_shifts<R>(a, b, sa, sb);
 #pragma unroll
             for (int t = 0; t < Q; ++t) {
                 win[0][mq(U)][t][0] = sa[t];
                 win[0][mq(U)][t][1] = sb[t];
             }
+        } else {
+            win[0][mq(U)][0][0] = a;
+            win[0][mq(U)][0][1] = b;
         }
         level<U, 0>(i);
     }
@@ -511,7 +540,7 @@ struct Strip2 {
         const int i0 = ys - R * L;
         const int iend = ye - 1 + R * L;
 #pragma unroll
-        for (int t = 0; t < D; ++t) load(i0 + t, raw[t]);
+        for (int t = 0; t < D; ++t) fl[t] = load(i0 + t);
         for (int i = i0; trip<0>(i, iend); i += UF) {
         }
     }

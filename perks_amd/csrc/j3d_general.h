@@ -18,5 +18,8 @@ int j3g_plan(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, con
              int occupancy, perks_plan_info* out);
 // Time steps per pass the persistent method uses for this shape and format.
 int j3g_levels(int dtype, int shape, const perks_stencil_opts* o);
+// The kernel o selects (opts.tune[4]: register streaming or LDS planes), its
+// workgroup size and dynamic LDS bytes, for attribute and occupancy queries.
+void* j3g_fn(int dtype, const perks_stencil_opts* o, int* block, size_t* lds);
 
 }  // namespace perks

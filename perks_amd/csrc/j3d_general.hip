@@ -164,7 +164,7 @@ int j3g_plan(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, con
         out->reserved[1] = nzs;
     } else {
         if (occupancy <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
-        int occ = occupancy;
+        int occ = std::min(occupancy, std::max(1, kCoopMaxWavesPerCu / (j3g::kThreads / 64)));
         if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
         if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
         const int G = caps->cu_count * occ;
@@ -175,6 +175,16 @@ int j3g_plan(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, con
     }
     out->grid[1] = out->grid[2] = 1;
     return PERKS_OK;
+}
+
+void* j3g_fn(int dtype, const perks_stencil_opts* o, int* block, size_t* lds)
+{
+    const ShapeDesc* sd = shape_by_id(o->shape);
+    if (!sd || sd->dims != 3) return nullptr;
+    const bool streaming = use_stream(o, sd->radius);
+    if (block) *block = streaming ? j3s::kWaves * 64 : j3g::kThreads;
+    if (lds) *lds = streaming ? 0 : lds_for(dtype, sd->radius, j3g_levels(dtype, o->shape, o));
+    return streaming ? stream_kernel_for(dtype, o->shape, o->boundary) : kernel_for(dtype, o->shape, o->boundary);
 }
 
 int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,

@@ -26,4 +26,9 @@ int launch_j3d_star1_plain(const T* in, T* out, int nz, int ny, int nx, const KC
 int launch_j3d_star1_plain16(int fmt, const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const KCoefs& cf,
                              int bc, int zchunk, bool sr, uint32_t key, hipStream_t stream, PlainGeom* geom);
 
+// The kernels the launchers above run (256-thread workgroups), for attribute
+// and occupancy queries.
+void* j3d_star1_plain_fn(int dtype, int bc, bool has_div, bool nontemporal);
+void* j3d_star1_plain16_fn(int fmt, int bc, bool has_div, bool sr);
+
 }  // namespace perks

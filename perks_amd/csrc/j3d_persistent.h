@@ -17,4 +17,10 @@ int run_persistent16(int fmt, uint16_t* d_a, uint16_t* d_b, int nz, int ny, int 
 
 int release_persistent_scratch();
 
+// Number of tile variants (opts.tune[4]) of the persistent 7-point kernel.
+int j3d_persistent_variants();
+// The persistent kernel of tile variant `var`, and its workgroup size in
+// *block, for attribute and occupancy queries; nullptr when not built.
+void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, int* block);
+
 }  // namespace perks

@@ -73,6 +73,19 @@ int variant_of(const perks_stencil_opts* o)
     return (v >= 0 && v < kNumVariants) ? v : 0;
 }
 
+// Time steps per pass when opts.tune[2] leaves it open, measured at 512^3 x
+// 1000 steps on MI355X (variant 0, zero boundary, GCells/s at 3 / 4 levels):
+// FP64 701 / 741 and FP32 1265 / 1445 take 4; FP16 1416 / 1290 (stochastic
+// rounding 731 / 657) and BF16 1320 / 1250 take 3, where the arithmetic of
+// the ghost rim outweighs the traffic a fourth level saves; BF16 with
+// stochastic rounding (904 / 947) takes 4.
+int default_levels(int dtype, bool sr)
+{
+    if (dtype == PERKS_F16) return 3;
+    if (dtype == PERKS_BF16) return sr ? 4 : 3;
+    return kMaxLevels;
+}
+
 // Columns a tile stores with l steps per pass: 64 - 2l, or for the
 // two-column variant 128 minus a rim of l rounded up to even per side.
 int tile_width(const Variant& v, int l) { return v.pair ? 128 - 4 * ((l + 1) / 2) : 64 - 2 * l; }
@@ -118,7 +131,7 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
     const int G = caps->cu_count * occ;
     if (G <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
-    int L = o->tune[2] > 0 ? o->tune[2] : kMaxLevels;
+    int L = o->tune[2] > 0 ? o->tune[2] : default_levels(dtype, o->rounding == PERKS_ROUND_STOCHASTIC);
     if (L > kMaxLevels) L = kMaxLevels;
     std::memset(out, 0, sizeof(*out));
     out->grid[0] = G;
@@ -167,6 +180,15 @@ int min_height_persistent(int dtype, int ny, int nx, const perks_stencil_opts*)
 int min_width_y_persistent(int dtype, int nx, int ny, int, int)
 {
     return min_height_persistent(dtype, ny, nx, nullptr);
+}
+
+int j3d_persistent_variants() { return kNumVariants; }
+
+void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, int* block)
+{
+    if (var < 0 || var >= kNumVariants) return nullptr;
+    if (block) *block = variant_shape(dtype, var).nw * 64;
+    return pick_kernel(dtype, bc, has_div, var, sr);
 }
 
 // ---- launch -----------------------------------------------------------------

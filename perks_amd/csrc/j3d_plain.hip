@@ -185,6 +185,33 @@ int launch_j3d_star1_plain(const T* in, T* out, int nz, int ny, int nx, const KC
     return launch_plain_ny<T, PERKS_BC_CLAMP, 8, false>(in, out, nz, ny, nx, cf, has_div, zchunk, stream, geom);
 }
 
+namespace {
+
+template <typename T, int BC>
+void* plain_fn(bool has_div, bool nt)
+{
+    if (has_div)
+        return nt ? reinterpret_cast<void*>(&j3d_star1_plain_kernel<T, BC, 8, true, true>)
+                  : reinterpret_cast<void*>(&j3d_star1_plain_kernel<T, BC, 8, true, false>);
+    return nt ? reinterpret_cast<void*>(&j3d_star1_plain_kernel<T, BC, 8, false, true>)
+              : reinterpret_cast<void*>(&j3d_star1_plain_kernel<T, BC, 8, false, false>);
+}
+
+template <typename T>
+void* plain_fn_bc(int bc, bool has_div, bool nt)
+{
+    return bc == PERKS_BC_ZERO ? plain_fn<T, PERKS_BC_ZERO>(has_div, nt) : plain_fn<T, PERKS_BC_CLAMP>(has_div, nt);
+}
+
+}  // namespace
+
+void* j3d_star1_plain_fn(int dtype, int bc, bool has_div, bool nontemporal)
+{
+    if (dtype == PERKS_F64) return plain_fn_bc<double>(bc, has_div, nontemporal);
+    if (dtype == PERKS_F32) return plain_fn_bc<float>(bc, has_div, nontemporal);
+    return nullptr;
+}
+
 template int launch_j3d_star1_plain<double>(const double*, double*, int, int, int, const KCoefs&, int, int,
                                             hipStream_t, PlainGeom*, int);
 template int launch_j3d_star1_plain<float>(const float*, float*, int, int, int, const KCoefs&, int, int,

@@ -195,6 +195,33 @@ int launch16_sr(bool sr, const uint16_t* in, uint16_t* out, int nz, int ny, int 
 
 }  // namespace
 
+namespace {
+
+template <int FMT, int BC>
+void* fn16(bool has_div, bool sr)
+{
+    if (has_div)
+        return sr ? reinterpret_cast<void*>(&j3d_star1_plain16_kernel<FMT, BC, 8, true, true>)
+                  : reinterpret_cast<void*>(&j3d_star1_plain16_kernel<FMT, BC, 8, true, false>);
+    return sr ? reinterpret_cast<void*>(&j3d_star1_plain16_kernel<FMT, BC, 8, false, true>)
+              : reinterpret_cast<void*>(&j3d_star1_plain16_kernel<FMT, BC, 8, false, false>);
+}
+
+template <int FMT>
+void* fn16_bc(int bc, bool has_div, bool sr)
+{
+    return bc == PERKS_BC_ZERO ? fn16<FMT, PERKS_BC_ZERO>(has_div, sr) : fn16<FMT, PERKS_BC_CLAMP>(has_div, sr);
+}
+
+}  // namespace
+
+void* j3d_star1_plain16_fn(int fmt, int bc, bool has_div, bool sr)
+{
+    if (fmt == PERKS_BF16) return fn16_bc<PERKS_BF16>(bc, has_div, sr);
+    if (fmt == PERKS_F16) return fn16_bc<PERKS_F16>(bc, has_div, sr);
+    return nullptr;
+}
+
 int launch_j3d_star1_plain16(int fmt, const uint16_t* in, uint16_t* out, int nz, int ny, int nx, const KCoefs& cf,
                              int bc, int zchunk, bool sr, uint32_t key, hipStream_t stream, PlainGeom* geom)
 {

@@ -6,7 +6,9 @@
 // by one wave, one register per row) over a z-slab; each wave runs its own
 // items, so there is no LDS and no workgroup barrier.  The wave marches along
 // z one plane per iteration: level 0 is the field plane read from HBM
-// (prefetched D planes ahead), and level l (1..L) computes plane i - R*l from
+// (requested D planes ahead and kept as loaded until it enters the window,
+// where it is widened, so nothing waits on a load before its first use), and
+// level l (1..L) computes plane i - R*l from
 // the 2R+1 planes of level l-1 it keeps in a register ring.  y neighbours are
 // the wave's own rows; x neighbours come from the adjacent lanes by DPP
 // whole-wave shifts (a shifted value is formed once per plane and reused by
@@ -16,8 +18,8 @@
 //
 // Boundaries: at every level the cells just outside the grid hold what the
 // boundary rule reads there -- 0 (zero) or the nearest edge cell (clamp) --
-// so the update itself never tests for a face.  Level 0 gets them from the
-// load (clamped indices, or 0); higher levels copy the edge row, column
+// so the update itself never tests for a face.  Level 0 gets them from
+// clamped loads (zeroed as the plane enters the window); higher levels copy the edge row, column
 // (readlane) or plane into the cells beyond it when it is produced.  The
 // update is the shape table in the gold's order without FMA contraction
 // (eval3) and every level is rounded to the storage format, so a pass equals
@@ -75,8 +77,7 @@ struct Tile3 {
     static constexpr int R = SD.radius;
     static constexpr int Q = 2 * R + 1;  // planes of a level's window
     static constexpr int D = Q;          // level-0 planes loaded ahead
-    static constexpr int P = Q + D;      // level-0 ring slots
-    static constexpr int UF = P;         // stages per loop trip (a multiple of Q: ring slots are static)
+    static constexpr int UF = D;         // stages per loop trip (a multiple of Q: ring slots are static)
     static constexpr int NY = kRows;
 
     const S* src;
@@ -99,30 +100,37 @@ struct Tile3 {
     bool hd, sr;
     uint32_t key[L + 1];
 
-    C raw[P][NY];     // level-0 planes as loaded
-    C win[L][Q][NY];  // level l's window (l >= 1)
+    using Raw = typename Fmt<S>::Raw;
+    Raw fl[D][NY];    // level-0 planes in flight, as loaded (plane i in slot (i - i0) mod D)
+    C win[L][Q][NY];  // level l's window
 
-    static constexpr int mp(int a) { return ((a % P) + P) % P; }
     static constexpr int mq(int a) { return ((a % Q) + Q) % Q; }
 
-    __device__ __forceinline__ void load(int q, C (&dstp)[NY]) const
+    __device__ __forceinline__ void load(int q, Raw (&dstp)[NY]) const
     {
         const int qc = q < 0 ? 0 : (q >= nz ? nz - 1 : q);
         const S* base = src + (size_t)qc * plane;
+#pragma unroll
+        for (int r = 0; r < NY; ++r) dstp[r] = Fmt<S>::ld_raw(base, (size_t)roff[r]);
+    }
+
+    // Plane q of level 0 enters the window at its first use, D planes after
+    // its load was issued: widened, and 0 outside the grid (zero boundary).
+    __device__ __forceinline__ void arrive(int q, const Raw (&w)[NY], C (&out)[NY]) const
+    {
         const bool qin = q >= 0 && q < nz;
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
-            const C v = Fmt<S>::ld(base, (size_t)roff[r]);
-            if constexpr (BC == PERKS_BC_ZERO) dstp[r] = (qin && rin[r] && !xout) ? v : (C)0;
-            else dstp[r] = v;
+            const C v = Fmt<S>::widen(w[r]);
+            if constexpr (BC == PERKS_BC_ZERO) out[r] = (qin && rin[r] && !xout) ? v : (C)0;
+            else out[r] = v;
         }
     }
 
     template <int LV>
     __device__ __forceinline__ C at(int rel, int r) const
     {
-        if constexpr (LV == 0) return raw[mp(rel)][r];
-        else return win[LV][mq(rel)][r];
+        return win[LV][mq(rel)][r];
     }
 
     // Level LV1 = LV + 1 computes plane q = i - R*LV1 from level LV's window.
@@ -208,12 +216,19 @@ struct Tile3 {
         }
     }
 
-    // Iteration i (stage U = (i - i0) mod UF): level-0 plane i has arrived;
-    // plane i + D is requested into the slot plane i - 2R - 1 held.
+    // Iteration i (stage U = (i - i0) mod UF): level-0 plane i, loaded D
+    // iterations ago, enters the window, and plane i + D is requested into
+    // its in-flight slot.
     template <int U>
     __device__ __forceinline__ void stage(int i)
     {
-        load(i + D, raw[mp(U + D)]);
+        // The next load is issued before plane i is waited for, so D planes
+        // stay in flight across the wait.
+        Raw w[NY];
+#pragma unroll
+        for (int r = 0; r < NY; ++r) w[r] = fl[U][r];
+        load(i + D, fl[U]);
+        arrive(i, w, win[0][mq(U)]);
         level<U, 0>(i);
     }
 
@@ -257,7 +272,7 @@ struct Tile3 {
         const int i0 = zs - R * L;
         const int iend = ze - 1 + R * L;
 #pragma unroll
-        for (int t = 0; t < D; ++t) load(i0 + t, raw[t]);
+        for (int t = 0; t < D; ++t) load(i0 + t, fl[t]);
         for (int i = i0; trip<0>(i, iend); i += UF) {
         }
     }

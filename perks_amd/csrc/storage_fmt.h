@@ -22,9 +22,15 @@ struct bf16s {
 
 template <typename S>
 struct Fmt;
+// Raw is the register form of a loaded storage word: a kernel that keeps
+// loads in flight holds Raw values and widens each when it is first used, so
+// the load's latency overlaps the work in between.
 template <>
 struct Fmt<double> {
     using C = double;
+    using Raw = double;
+    __device__ static Raw ld_raw(const double* p, size_t i) { return p[i]; }
+    __device__ static C widen(Raw w) { return w; }
     __device__ static C ld(const double* p, size_t i) { return p[i]; }
     __device__ static void st(double* p, size_t i, C v) { p[i] = v; }
     __device__ static C q(C v) { return v; }
@@ -33,6 +39,9 @@ struct Fmt<double> {
 template <>
 struct Fmt<float> {
     using C = float;
+    using Raw = float;
+    __device__ static Raw ld_raw(const float* p, size_t i) { return p[i]; }
+    __device__ static C widen(Raw w) { return w; }
     __device__ static C ld(const float* p, size_t i) { return p[i]; }
     __device__ static void st(float* p, size_t i, C v) { p[i] = v; }
     __device__ static C q(C v) { return v; }
@@ -41,6 +50,9 @@ struct Fmt<float> {
 template <>
 struct Fmt<f16s> {
     using C = float;
+    using Raw = uint32_t;
+    __device__ static Raw ld_raw(const f16s* p, size_t i) { return p[i].v; }
+    __device__ static C widen(Raw w) { return __half2float(__ushort_as_half((uint16_t)w)); }
     __device__ static C ld(const f16s* p, size_t i) { return __half2float(__ushort_as_half(p[i].v)); }
     __device__ static void st(f16s* p, size_t i, C v) { p[i].v = __half_as_ushort(__float2half_rn(v)); }
     // The value after a round trip through storage (every step is stored).
@@ -54,6 +66,9 @@ struct Fmt<f16s> {
 template <>
 struct Fmt<bf16s> {
     using C = float;
+    using Raw = uint32_t;
+    __device__ static Raw ld_raw(const bf16s* p, size_t i) { return p[i].v; }
+    __device__ static C widen(Raw w) { return __uint_as_float(w << 16); }
     __device__ static C ld(const bf16s* p, size_t i) { return __uint_as_float((uint32_t)p[i].v << 16); }
     // Round to nearest even; NaN stays a quiet NaN.
     __device__ static uint16_t bits(C v) { return (uint16_t)Store16<PERKS_BF16>::narrow(v); }

@@ -164,6 +164,49 @@ def j3d_device(d_a, d_b, height, width_y, width_x, steps, dtype="f64", stream=No
     return rep
 
 
+def _config_rows(table, n) -> list[dict]:
+    rows = []
+    for i in range(n):
+        t = table[i]
+        rows.append({"method": "plain" if t.method == L.METHOD_PLAIN else "persistent", "variant": t.variant,
+                     "levels": t.levels, "block": t.block, "vgprs": t.vgprs, "spill_bytes": t.spill_bytes,
+                     "lds_bytes": t.lds_bytes, "occupancy": t.occupancy, "tile": (t.tile_x, t.tile_y),
+                     "status": t.status, "gcells_per_s": t.gcells_per_s})
+    return rows
+
+
+def j3d_kernel_table(height, width_y, width_x, dtype="f64", opts=None, **kw) -> list[dict]:
+    """Every launch configuration of the 3D path for this problem, with what
+    its code object reports (VGPRs, spill bytes, LDS, occupancy) and its
+    status (0, or the error code saying why it cannot run)."""
+    lib = L.load()
+    o = opts if opts is not None else make_opts(**kw)
+    n = C.c_int(0)
+    L.check(lib.perks_j3d_kernel_table(_dt(dtype), height, width_y, width_x, C.byref(o), None, 0, C.byref(n)),
+            "j3d_kernel_table")
+    table = (L.KernelConfig * max(1, n.value))()
+    L.check(lib.perks_j3d_kernel_table(_dt(dtype), height, width_y, width_x, C.byref(o), table, n.value,
+                                       C.byref(n)), "j3d_kernel_table")
+    return _config_rows(table, n.value)
+
+
+def j3d_autotune(d_in, height, width_y, width_x, dtype="f64", sample_steps=0, stream=None, opts=None,
+                 **kw) -> tuple[L.StencilOpts, list[dict]]:
+    """Times every runnable configuration on the field d_in (left unchanged)
+    and returns the options with the fastest one selected (method, tune[2],
+    tune[4]) and the table with the measured rates."""
+    lib = L.load()
+    o = opts if opts is not None else make_opts(**kw)
+    n = C.c_int(0)
+    L.check(lib.perks_j3d_kernel_table(_dt(dtype), height, width_y, width_x, C.byref(o), None, 0, C.byref(n)),
+            "j3d_kernel_table")
+    table = (L.KernelConfig * max(1, n.value))()
+    s = None if stream is None else _ptr_stream(stream)
+    L.check(lib.perks_j3d_autotune(_dt(dtype), _ptr(d_in), height, width_y, width_x, int(sample_steps),
+                                   C.byref(o), table, n.value, C.byref(n), s), "j3d_autotune")
+    return o, _config_rows(table, n.value)
+
+
 def _ptr_stream(stream) -> int:
     if isinstance(stream, int):
         return stream

@@ -35,7 +35,8 @@ def test_ctypes_mirror_covers_the_header():
 
 
 STRUCTS = {"perks_stencil_opts": L.StencilOpts, "perks_run_report": L.RunReport, "perks_grid2d": L.Grid2d,
-           "perks_device_caps": L.DeviceCaps, "perks_plan_info": L.PlanInfo, "perks_csr": L.Csr, "perks_dia": L.Dia,
+           "perks_device_caps": L.DeviceCaps, "perks_plan_info": L.PlanInfo,
+           "perks_kernel_config": L.KernelConfig, "perks_csr": L.Csr, "perks_dia": L.Dia,
            "perks_cg_opts": L.CgOpts, "perks_cg_report": L.CgReport}
 
 
