@@ -102,7 +102,8 @@ typedef struct perks_stencil_opts {
      *           (3D 7-point default: 4; 3 for FP16 and for BF16 with nearest rounding;
      *           perks_j3d_autotune measures every choice)
      *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel),
-     *           4 = two columns per lane (FP32/FP16/BF16, even widths).
+     *           4 = two columns per lane (FP32/FP16/BF16, even widths) or, for FP64,
+     *           8 waves x 5 rows.
      *           3D other shapes: 0 = default (2), 2 = register streaming, 3 = LDS planes.
      *           2D: 0 = default (4 where it applies, else 2), 1 = LDS tile 128x64 (64x64 FP64),
      *           2 = row streaming through registers, one column per lane, 3 = LDS tile 128x32

@@ -98,10 +98,10 @@ int tile_width(const Variant& v, int l) { return v.pair ? 128 - 4 * ((l + 1) / 2
 // G workgroups; an item costs its slab height plus the 2*l ghost planes it
 // reads beyond the slab.  Picks the z-slab count that minimises
 // rounds * (planes per item).
-static int best_nzs(int nz, int ny, int nx, int l, int G, int rows, int VW)
+static int best_nzs(int nz, int ny, int ntx, int l, int G, int rows)
 {
     const int VH = rows - 2 * (l - 1);
-    const long long tiles = (long long)((nx + VW - 1) / VW) * ((ny + VH - 1) / VH);
+    const long long tiles = (long long)ntx * ((ny + VH - 1) / VH);
     int best = 1;
     double best_cost = 1e300;
     const int cap = nz < 256 ? nz : 256;
@@ -144,7 +144,12 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     out->tile_y = v.nw * v.ny;
     out->reserved[0] = L;
     for (int l = 1; l <= kMaxLevels; ++l)
-        out->reserved[l] = best_nzs(nz, ny, nx, l, G, v.nw * v.ny, tile_width(v, l));
+    {
+        // x tiles as the kernel cuts them (j3dp::x_tiles; the two-column form
+        // keeps ghost lanes at both faces)
+        const int ntx = v.pair ? (nx + tile_width(v, l) - 1) / tile_width(v, l) : j3dp::x_tiles(o->boundary, nx, l);
+        out->reserved[l] = best_nzs(nz, ny, ntx, l, G, v.nw * v.ny);
+    }
     return PERKS_OK;
 }
 

@@ -83,6 +83,51 @@ struct Elem<PERKS_F16> : Elem16<PERKS_F16> {};
 template <>
 struct Elem<PERKS_BF16> : Elem16<PERKS_BF16> {};
 
+// Tiling of the x axis for L levels (one column per lane).  A tile's 64
+// lanes store 64 - 2L columns: each level's valid region shrinks by a lane
+// per side.  Under the zero boundary the cells beyond an x face are 0 at
+// every level, which is exactly what the DPP shifts bring into lanes 0 and
+// 63, so a tile whose first (last) lane is the face column loses no lane on
+// that side: the two edge tiles start at the faces and store 64 - L columns
+// each.  At 512 columns and L = 4 that is 60 + 7 x 56 + 60, nine tiles
+// instead of ten.
+__host__ __device__ inline bool x_face_tiles(int bc, int nx, int L)
+{
+    return bc == PERKS_BC_ZERO && nx >= 2 * (64 - L);
+}
+__host__ __device__ inline int x_tiles(int bc, int nx, int L)
+{
+    const int VW = 64 - 2 * L;
+    if (!x_face_tiles(bc, nx, L)) return (nx + VW - 1) / VW;
+    const int EW = 64 - L;
+    return 2 + (nx - 2 * EW + VW - 1) / VW;
+}
+// Tile tx of ntx: stored columns [vx0, vx1) and the column of lane 0.
+__host__ __device__ inline void x_tile(int bc, int nx, int L, int tx, int ntx, int& vx0, int& vx1, int& xw0)
+{
+    const int VW = 64 - 2 * L;
+    if (!x_face_tiles(bc, nx, L)) {
+        vx0 = tx * VW;
+        vx1 = vx0 + VW < nx ? vx0 + VW : nx;
+        xw0 = vx0 - L;
+        return;
+    }
+    const int EW = 64 - L;
+    if (tx == 0) {
+        vx0 = 0;
+        vx1 = EW;
+        xw0 = 0;
+    } else if (tx == ntx - 1) {
+        vx0 = nx - EW;
+        vx1 = nx;
+        xw0 = nx - 64;
+    } else {
+        vx0 = EW + (tx - 1) * VW;
+        vx1 = vx0 + VW < nx - EW ? vx0 + VW : nx - EW;
+        xw0 = vx0 - L;
+    }
+}
+
 // One work item: L time steps over an x-y tile and a z-slab.
 //
 // Lanes 0 and 63 are ghost lanes at level 0 already (no halo column loads):
@@ -321,9 +366,8 @@ struct March {
     {
         // The exchange rows of the previous item may still be being read.
         lds_barrier();
-        constexpr int VW = 64 - 2 * L;
         const int VH = NW * NY - 2 * (L - 1);
-        const int ntx = (nx + VW - 1) / VW;
+        const int ntx = x_tiles(BC, nx, L);
         const int nty = (ny + VH - 1) / VH;
         const int tx = item % ntx;
         const int ty = (item / ntx) % nty;
@@ -331,9 +375,9 @@ struct March {
         const int zs = (int)(((long long)zsl * nz) / nzs);
         const int ze = (int)(((long long)(zsl + 1) * nz) / nzs);
 
-        const int vx0 = tx * VW, vx1 = min(vx0 + VW, nx);
+        int vx0, vx1, xw0;  // stored columns, column of lane 0
+        x_tile(BC, nx, L, tx, ntx, vx0, vx1, xw0);
         const int vy0 = ty * VH, vy1 = min(vy0 + VH, ny);
-        const int xw0 = vx0 - L;  // column of lane 0
         const int x = xw0 + lane;
         const int xc = x < 0 ? 0 : (x >= nx ? nx - 1 : x);
         const int y0 = vy0 - (L - 1) + wave * NY;
@@ -765,9 +809,8 @@ __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typen
                                          typename Elem<FMT>::T* xch, const PersistArgs& a,
                                          const typename Elem<FMT>::T* k, int lane, int wave, int step0)
 {
-    constexpr int VW = 64 - 2 * L;
     const int VH = NW * NY - 2 * (L - 1);
-    const int nitems = ((a.nx + VW - 1) / VW) * ((a.ny + VH - 1) / VH) * a.nzs[L];
+    const int nitems = x_tiles(BC, a.nx, L) * ((a.ny + VH - 1) / VH) * a.nzs[L];
     const int G = gridDim.x;
     const int me = xcd_tile_id(blockIdx.x, G);
     for (int item = me; item < nitems; item += G) {
@@ -833,16 +876,17 @@ j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S*
 //   1  8 waves x 4 rows, 9 slots                  as 0 with 9 slots
 //   2  8 waves x 4 rows (two workgroups per CU)   4 waves x 4 rows (two per CU)
 //   3  16 waves x 2 rows                          16 waves x 2 rows
-//   4  8 waves x 4 rows, two columns per lane     (as 0)
-//      (a 128 x 32 tile; even grid widths; at 512^3 5% slower than 0 for
-//      FP32 and 2% for BF16: twice the registers halve the occupancy)
+//   4  8 waves x 4 rows, two columns per lane     8 waves x 5 rows (64 x 40 tile)
+//      (FP32: a 128 x 32 tile; even grid widths; at 512^3 5% slower than 0
+//      for FP32 and 2% for BF16: twice the registers halve the occupancy.
+//      FP64: 230 VGPRs; 6 rows would need more than 256 and spill)
 // Selected through opts->tune[4].
 struct Variant {
     int ny, nw, p, wpe;
     int pair;  // two columns per lane (March2)
 };
 constexpr Variant kVarF32[] = {{4, 16, 6, 1, 0}, {4, 8, 9, 1, 0}, {4, 8, 6, 1, 0}, {2, 16, 6, 1, 0}, {4, 8, 6, 1, 1}};
-constexpr Variant kVarF64[] = {{4, 8, 6, 1, 0}, {4, 8, 9, 1, 0}, {4, 4, 6, 1, 0}, {2, 16, 6, 1, 0}, {4, 8, 6, 1, 0}};
+constexpr Variant kVarF64[] = {{4, 8, 6, 1, 0}, {4, 8, 9, 1, 0}, {4, 4, 6, 1, 0}, {2, 16, 6, 1, 0}, {5, 8, 6, 1, 0}};
 constexpr int kNumVariants = 5;
 constexpr int kPairVariant = 4;
 __host__ __device__ constexpr Variant variant_shape(int fmt, int v)

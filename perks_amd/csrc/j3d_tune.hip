@@ -59,9 +59,6 @@ std::vector<Candidate> candidates(int dtype, const perks_stencil_opts* o)
         c.push_back({PERKS_METHOD_PLAIN, 0, 1});
         const int nv = j3d_persistent_variants();
         for (int v = 0; v < nv; ++v) {
-            // Two columns per lane exists for FP32 arithmetic only; for FP64
-            // that slot holds a copy of variant 0.
-            if (dtype == PERKS_F64 && v == nv - 1) continue;
             for (int l = 1; l <= 4; ++l) c.push_back({PERKS_METHOD_PERSISTENT, v, l});
         }
         return c;

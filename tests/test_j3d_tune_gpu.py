@@ -41,8 +41,7 @@ def test_table_lists_every_configuration_and_none_spills(gpu, shape, dtype):
     methods = {r["method"] for r in rows}
     assert methods == {"plain", "persistent"}
     if shape == "3d7pt":
-        variants = 4 if dtype == "f64" else 5
-        assert len(rows) == 1 + 4 * variants
+        assert len(rows) == 1 + 4 * 5
         assert sorted({r["levels"] for r in rows if r["method"] == "persistent"}) == [1, 2, 3, 4]
     else:
         assert {r["variant"] for r in rows} == {2, 3}
