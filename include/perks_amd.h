@@ -99,8 +99,8 @@ typedef struct perks_stencil_opts {
     /* experimental knobs, 0 = automatic:
      *   tune[0] plain: z-chunk per workgroup       tune[1] plain: 2 = cached stores (default nontemporal)
      *   tune[2] persistent: time steps per pass     tune[3] persistent: workgroups per CU
-     *           (3D 7-point default: 4; 3 for FP16 and for BF16 with nearest rounding;
-     *           perks_j3d_autotune measures every choice)
+     *           (3D 7-point default: 4; under the clamp boundary 3 for FP16 and for BF16
+     *           with nearest rounding; perks_j3d_autotune measures every choice)
      *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel),
      *           4 = two columns per lane (FP32/FP16/BF16, even widths) or, for FP64,
      *           8 waves x 5 rows.

@@ -74,16 +74,21 @@ int variant_of(const perks_stencil_opts* o)
 }
 
 // Time steps per pass when opts.tune[2] leaves it open, measured at 512^3 x
-// 1000 steps on MI355X (variant 0, zero boundary, GCells/s at 3 / 4 levels):
-// FP64 701 / 741 and FP32 1265 / 1445 take 4; FP16 1416 / 1290 (stochastic
-// rounding 731 / 657) and BF16 1320 / 1250 take 3, where the arithmetic of
-// the ghost rim outweighs the traffic a fourth level saves; BF16 with
-// stochastic rounding (904 / 947) takes 4.
-int default_levels(int dtype, bool sr)
+// 1000 steps on MI355X (variant 0, GCells/s at 3 / 4 levels).  FP64 and FP32
+// take 4 (zero boundary: FP64 701 / 741, FP32 1265 / 1445 with ghost lanes at
+// both faces; more since the edge tiles start at the faces).  16-bit storage:
+// where the x edge tiles start at the faces (zero boundary, wide grids) a
+// fourth level saves a tile column as well as traffic and 4 is fastest (BF16
+// 1340 / 1399, FP16 1414 / 1485, stochastic rounding BF16 1025 / 1134, FP16
+// 747 / 754); with ghost lanes at both faces (clamp) FP16 (1416 / 1290; with
+// stochastic rounding 731 / 657) and BF16 with nearest rounding (1320 / 1250)
+// take 3, where the ghost rim's arithmetic outweighs the traffic saved.
+int default_levels(int dtype, bool sr, int bc, int nx)
 {
+    if (dtype == PERKS_F64 || dtype == PERKS_F32) return kMaxLevels;
+    if (j3dp::x_face_tiles(bc, nx, kMaxLevels)) return kMaxLevels;
     if (dtype == PERKS_F16) return 3;
-    if (dtype == PERKS_BF16) return sr ? 4 : 3;
-    return kMaxLevels;
+    return sr ? kMaxLevels : 3;
 }
 
 // Columns a tile stores with l steps per pass: 64 - 2l, or for the
@@ -131,7 +136,8 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     if (o->tune[3] > 0 && o->tune[3] < occ) occ = o->tune[3];
     const int G = caps->cu_count * occ;
     if (G <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
-    int L = o->tune[2] > 0 ? o->tune[2] : default_levels(dtype, o->rounding == PERKS_ROUND_STOCHASTIC);
+    int L = o->tune[2] > 0 ? o->tune[2]
+                           : default_levels(dtype, o->rounding == PERKS_ROUND_STOCHASTIC, o->boundary, nx);
     if (L > kMaxLevels) L = kMaxLevels;
     std::memset(out, 0, sizeof(*out));
     out->grid[0] = G;
