@@ -362,7 +362,7 @@ struct March {
         return true;
     }
 
-    __device__ void run(int item, int nzs)
+    __device__ __forceinline__ void run(int item, int nzs)
     {
         // The exchange rows of the previous item may still be being read.
         lds_barrier();
@@ -687,7 +687,7 @@ struct March2 {
         return true;
     }
 
-    __device__ void run(int item, int nzs)
+    __device__ __forceinline__ void run(int item, int nzs)
     {
         lds_barrier();
         constexpr int VW = 128 - 2 * G;

@@ -48,16 +48,13 @@ def test_table_lists_every_configuration_and_none_spills(gpu, shape, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["f16", "bf16"])
-def test_stochastic_rounding_table_spills_only_in_the_two_column_variant(gpu, dtype):
-    """With stochastic rounding the two-columns-per-lane 7-point kernel keeps
-    scratch: the table must say so and the autotuner must not pick it."""
+def test_stochastic_rounding_table_has_no_spills(gpu, dtype):
+    """Stochastic rounding makes the largest kernel bodies (the two-column
+    7-point variant at four levels once kept its whole march object in scratch
+    when the compiler declined to inline it): every entry must be spill-free."""
     rows = P.j3d_kernel_table(64, 64, 64, dtype=dtype, shape="3d7pt", boundary="zero", rounding="stochastic")
     for r in rows:
-        pair = r["method"] == "persistent" and r["variant"] == 4
-        if pair:
-            assert r["spill_bytes"] > 0 and r["status"] == L.ERR_SPILL, r
-        else:
-            assert r["spill_bytes"] == 0 and r["status"] == L.PERKS_OK, r
+        assert r["spill_bytes"] == 0 and r["status"] == L.PERKS_OK, r
 
 
 def test_two_column_variant_refuses_odd_widths(gpu):
