@@ -11,8 +11,19 @@ Workload (one bench "step" = one complete solve, inputs already in HBM):
 
 Prints one JSON line (see the keys at the bottom).  GCells/s counts
 time steps * cells / seconds, the reference's own definition
-(stencil/3dstencil/j3d.cu:754); the roofline prices one read plus one write
-of the field per cell-update (j3d.cu:756) against 8 TB/s of HBM.
+(stencil/3dstencil/j3d.cu:754).  The roofline prices the algorithmic HBM
+bytes of the kernel's pass structure against 8 TB/s: a kernel that runs L
+time steps per pass must read and write the field once per pass, so a launch
+of T steps moves 2 * sizeof(T) * cells * ceil(T / L) bytes (L = 1 for the
+per-step kernel: the reference's single-pass model, j3d.cu:756).  The
+single-pass figure is reported beside it as roofline.single_pass_gbs.  The
+counter-measured traffic (FETCH_SIZE + WRITE_SIZE) and the VALU figures come
+from profiles/<kernel>_traffic.json, used only when the kernel sources it
+was recorded from are the ones in this tree (SHA-256 over KERNEL_SOURCES).
+
+    python bench.py --record-profile DIR_FETCH DIR_WRITE [DIR_SQ]
+reads the rocprofv3 counter CSVs of the default workload (one directory per
+counter pass) and writes that JSON.
 
 Other workloads (--workload; each prints its own JSON line):
   j3d-storage  the same 3D solve with FP64 / FP32 / FP16 / BF16 storage (FP32
@@ -49,6 +60,43 @@ METRIC = "GCells/s + % of HBM roofline, 3d7pt fp64 512³, persistent vs per-step
 HBM_PEAK_GBS = 8000.0
 CONFIG_COEFS = [0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4]  # E W N S T B, centre
 TRAFFIC_FILE = ROOT / "profiles" / "j3d_persistent_f64_512_traffic.json"
+PERSISTENT_KERNEL = "j3d_star1_persistent_kernel"
+# Files that define the persistent 7-point kernel and its launch: a recorded
+# profile applies only while their contents are unchanged.
+KERNEL_SOURCES = ["perks_amd/csrc/j3d_persistent_kernel.h", "perks_amd/csrc/j3d_persistent.hip",
+                  "perks_amd/csrc/j3d_persistent_f64.hip", "perks_amd/csrc/j3d_persistent.h",
+                  "perks_amd/csrc/grid_sync.h", "perks_amd/csrc/perks_device.h", "perks_amd/csrc/j3d_common.h",
+                  "perks_amd/csrc/storage16.h", "perks_amd/csrc/storage_fmt.h", "perks_amd/build.py"]
+SIMDS, SHADER_ENGINES = 1024, 32  # MI355X: 256 CUs x 4 SIMDs; 8 XCDs x 4 SEs
+
+
+def kernel_sources_sha() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        h.update(rel.encode())
+        h.update((ROOT / rel).read_bytes())
+    return h.hexdigest()
+
+
+def pass_roofline(kernel, cells, time_steps, steps_per_pass, esz, seconds, traffic=None):
+    """Roofline of one launch: algorithmic bytes = one read and one write of
+    the field per pass of `steps_per_pass` time steps.  `cells` and
+    `time_steps` may be lists (a batch of grids)."""
+    cl = cells if isinstance(cells, list) else [cells]
+    ts = time_steps if isinstance(time_steps, list) else [time_steps] * len(cl)
+    L = max(1, int(steps_per_pass))
+    alg = sum(2 * esz * c * (-(-t // L)) for c, t in zip(cl, ts))
+    single = sum(2 * esz * c * t for c, t in zip(cl, ts))
+    achieved = alg / seconds / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+           "algorithmic_bytes_per_launch": alg, "steps_per_pass": L,
+           "algorithmic_model": "one read + one write of the field per pass of steps_per_pass time steps",
+           "single_pass_gbs": round(single / seconds / 1e9, 1)}
+    if traffic:
+        out["traffic_gbs"] = round(traffic / seconds / 1e9, 1)
+    return out
 
 
 def parse():
@@ -67,6 +115,9 @@ def parse():
                                                   "j3d-tune"],
                     default="j3d")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--record-profile", nargs="+", metavar="DIR",
+                    help="write the counter profile of the default workload from rocprofv3 CSV directories: "
+                         "FETCH_SIZE pass, WRITE_SIZE pass[, SQ pass]")
     return ap.parse_args()
 
 
@@ -145,13 +196,16 @@ def copy_bandwidth(torch, like) -> float:
 
 
 def load_traffic(n, tsteps, dtype):
+    """The recorded counter profile of the persistent kernel, or (None, why)."""
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
-        return None
+        return None, "no recorded profile"
     if d.get("size") != n or d.get("time_steps") != tsteps or d.get("dtype") != dtype:
-        return None
-    return d.get("hbm_bytes_per_launch")
+        return None, "recorded profile is for another workload"
+    if d.get("source_sha256") != kernel_sources_sha():
+        return None, "recorded profile is for other kernel sources"
+    return d, None
 
 
 def run_j3d(args):
@@ -209,18 +263,17 @@ def run_j3d(args):
                                                                                    else torch.int32)))
 
     avg_launch_s = kern_s / args.steps
-    alg_bytes = T * cells * 2 * esz
-    achieved = alg_bytes / avg_launch_s / 1e9
-    traffic = load_traffic(n, T, args.dtype) if args.method == "persistent" else None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "j3d_star1_persistent_kernel" if args.method == "persistent" else
-                          "j3d_star1_plain_kernel",
-                "algorithmic_bytes_per_launch": alg_bytes}
-    # Second denominators: the counter-measured HBM traffic per launch over the
-    # launch time, and what a device-to-device copy of the field achieves here.
-    if traffic:
-        roofline["traffic_gbs"] = round(traffic / avg_launch_s / 1e9, 1)
+    prof, why = load_traffic(n, T, args.dtype) if args.method == "persistent" else (None, "per-step kernel")
+    roofline = pass_roofline(PERSISTENT_KERNEL if args.method == "persistent" else "j3d_star1_plain_kernel",
+                             cells, T, levels if args.method == "persistent" else 1, esz, avg_launch_s,
+                             prof["hbm_bytes_per_launch"] if prof else None)
+    if prof:
+        roofline["profile"] = {k: prof[k] for k in ("kernel", "git_commit", "valu_busy", "valu_lane_insts_per_cell",
+                                                     "avg_launch_ms") if k in prof}
+        if prof.get("valu_busy") is not None:
+            roofline["limiter"] = "valu"
+    else:
+        roofline["traffic_note"] = why
     roofline["copy_gbs"] = round(copy_bandwidth(torch, a), 1)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GCells/s", "n_gpus": 1,
@@ -283,8 +336,6 @@ def run_j3d_storage(args):
         for m in ("persistent", "plain"):
             rows[name][m]["speedup_vs_f32"] = round(rows[name][m]["gcells_per_s"] / rows["f32"][m]["gcells_per_s"], 3)
     v = rows["bf16_sr"]["plain"]
-    alg = T * cells * 2 * 2
-    achieved = alg / (v["ms_per_solve"] / 1e3) / 1e9
     out = {
         "metric": "GCells/s, 3d7pt 512^3 1000 steps: FP64 / FP32 / FP16 / BF16 storage, persistent and per-step, "
                   "error vs FP64",
@@ -295,10 +346,8 @@ def run_j3d_storage(args):
                                f"(seed {args.seed}), {T} time steps; value = BF16 storage with stochastic "
                                "rounding, per-step kernel",
                    "size": n, "time_steps": T, "results": rows},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "j3d_star1_plain16_kernel<bf16, stochastic> (x 1000 launches)",
-                     "algorithmic_bytes_per_launch": alg},
+        "roofline": pass_roofline("j3d_star1_plain16_kernel<bf16, stochastic> (x 1000 launches)", cells, T, 1, 2,
+                                  v["ms_per_solve"] / 1e3),
     }
     print(json.dumps(out), flush=True)
 
@@ -334,8 +383,6 @@ def run_j3d_shapes(args):
         del a, b, init
         torch.cuda.empty_cache()
     v = rows["3d27pt_f64"]["persistent"]
-    alg = T * cells * 2 * 8
-    achieved = alg / (v["ms_per_solve"] / 1e3) / 1e9
     out = {
         "metric": "GCells/s, every 3D shape at 512^3, persistent vs per-step", "value": v["gcells_per_s"],
         "unit": "GCells/s", "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": v["ms_per_solve"],
@@ -343,9 +390,8 @@ def run_j3d_shapes(args):
         "config": {"workload": f"3D Jacobi, every reference shape with its reference weights, {n}^3, clamp boundary, "
                                f"{T} time steps; value = 3d27pt FP64 persistent", "size": n, "time_steps": T,
                    "results": rows},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "j3s_kernel<3d27pt, double> (register streaming)", "algorithmic_bytes_per_launch": alg},
+        "roofline": pass_roofline("j3s_kernel<3d27pt, double> (register streaming)", cells, T, v["steps_per_pass"], 8,
+                                  v["ms_per_solve"] / 1e3),
     }
     print(json.dumps(out), flush=True)
 
@@ -461,8 +507,6 @@ def run_j2d_storage(args):
         results[shape] = per
     bf = results["2d9pt"]["bf16"]
     ms = kernel_ms[("2d9pt", "bf16")]
-    alg = T * cells * 2 * 2
-    achieved = alg / (ms / 1e3) / 1e9
     out = {
         "metric": "GCells/s, 2D box r1/r2 8192^2 500 steps, FP32 vs FP16 vs BF16 storage (FP32 arithmetic), "
                   "error vs FP64",
@@ -473,9 +517,7 @@ def run_j2d_storage(args):
                                f"uniform[0,1) normalised to sum 1 (seed {args.seed}), input uniform[-1,1), {T} steps, "
                                "persistent kernel; value = 2d9pt BF16", "size": n, "time_steps": T,
                    "results": results},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "j2d_stream_kernel<2d9pt, bf16s>", "algorithmic_bytes_per_launch": alg},
+        "roofline": pass_roofline("j2d_stream_kernel<2d9pt, bf16s>", cells, T, bf["steps_per_pass"], 2, ms / 1e3),
     }
     print(json.dumps(out), flush=True)
 
@@ -551,8 +593,6 @@ def run_batch2d(args):
         del bufs, grids, batched, seq
         torch.cuda.empty_cache()
     r = out_rows["2d5pt"]
-    alg = cell_steps * 2 * 4
-    achieved = alg / (r["ms_per_batch"] / 1e3) / 1e9
     out = {
         "metric": "GCells/s, 512 batched 2D grids (5pt/9pt FP32) in one autotuned, hipGraph-captured persistent launch "
                   "vs sequential per-step",
@@ -562,10 +602,9 @@ def run_batch2d(args):
         "config": {"workload": f"512 grids, sides log-uniform in [64,1024] (non-square), steps uniform in [50,500], "
                                f"FP32, zero boundary, equal weights, seed {args.seed}; value = 2d5pt",
                    "grids": len(specs), "cell_steps": cell_steps, "results": out_rows},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": ("j2d_stream_kernel" if r["autotuned_config"]["tile"] == 2 else "j2d_kernel")
-                               + "<2d5pt, float> (batched)", "algorithmic_bytes_per_launch": alg},
+        "roofline": pass_roofline(("j2d_stream_kernel" if r["autotuned_config"]["tile"] in (2, 4) else "j2d_kernel")
+                                  + "<2d5pt, float> (batched)", [wy * wx for wy, wx, _ in specs],
+                                  [st for _, _, st in specs], r["steps_per_pass"], 4, r["ms_per_batch"] / 1e3),
     }
     print(json.dumps(out), flush=True)
 
@@ -654,8 +693,84 @@ def run_cg(args):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------- recorded counter profile
+def _counter_rows(directory):
+    """(kernel name, dispatch id, counter, value) of every rocprofv3 counter
+    CSV under `directory`."""
+    import csv
+    out = []
+    files = sorted(pathlib.Path(directory).rglob("*counter_collection.csv"))
+    if not files:
+        raise SystemExit(f"no *counter_collection.csv under {directory}")
+    for f in files:
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                out.append((row["Kernel_Name"], (str(f), row.get("Dispatch_Id", "")), row["Counter_Name"],
+                            float(row["Counter_Value"])))
+    return out
+
+
+def _per_dispatch(rows, kernel_substr, counter):
+    """Average over the dispatches of a kernel of the counter's value (summed
+    over the rows of one dispatch)."""
+    acc = {}
+    for name, disp, cname, v in rows:
+        if kernel_substr in name and cname == counter:
+            acc[disp] = acc.get(disp, 0.0) + v
+    if not acc:
+        raise SystemExit(f"no {counter} rows for a kernel matching {kernel_substr!r}")
+    return sum(acc.values()) / len(acc), len(acc)
+
+
+def record_profile(args, out_path=None):
+    """Writes TRAFFIC_FILE from the counter passes of
+        rocprofv3 --pmc FETCH_SIZE -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-per-step
+    (then WRITE_SIZE, then the SQ counters).  The byte counters are scaled by
+    what they read for the run's own device-to-device copies of the field
+    (__amd_rocclr_copyBuffer, one read and one write of cells * sizeof(T))."""
+    import subprocess
+    dirs = args.record_profile
+    n, T = args.size, args.tsteps
+    esz = 8 if args.dtype == "f64" else 4
+    field = n ** 3 * esz
+    fetch_rows, write_rows = _counter_rows(dirs[0]), _counter_rows(dirs[1])
+    fetch, nd = _per_dispatch(fetch_rows, PERSISTENT_KERNEL, "FETCH_SIZE")
+    write, _ = _per_dispatch(write_rows, PERSISTENT_KERNEL, "WRITE_SIZE")
+    cf, _ = _per_dispatch(fetch_rows, "copyBuffer", "FETCH_SIZE")
+    cw, _ = _per_dispatch(write_rows, "copyBuffer", "WRITE_SIZE")
+    ff, wf = field / (cf * 1024), field / (cw * 1024)
+    traffic = int(round(fetch * 1024 * ff + write * 1024 * wf))
+    names = sorted({k for k, _, c, _ in fetch_rows if PERSISTENT_KERNEL in k})
+    try:
+        commit = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short=12", "HEAD"], capture_output=True,
+                                text=True).stdout.strip() or None
+    except OSError:
+        commit = None
+    d = {"kernel": names[0] if names else PERSISTENT_KERNEL, "size": n, "time_steps": T, "dtype": args.dtype,
+         "dispatches": nd, "fetch_size_kib": fetch, "write_size_kib": write,
+         "fetch_factor": round(ff, 4), "write_factor": round(wf, 4), "hbm_bytes_per_launch": traffic,
+         "calibration": f"__amd_rocclr_copyBuffer of the {field}-byte field in the same runs: FETCH_SIZE {cf:.1f} KiB, "
+                        f"WRITE_SIZE {cw:.1f} KiB",
+         "source_sha256": kernel_sources_sha(), "git_commit": commit,
+         "command": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE, then SQ_INSTS_VALU SQ_ACTIVE_INST_VALU "
+                    "SQ_BUSY_CYCLES) -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-per-step"}
+    if len(dirs) > 2:
+        sq = _counter_rows(dirs[2])
+        insts, _ = _per_dispatch(sq, PERSISTENT_KERNEL, "SQ_INSTS_VALU")
+        active, _ = _per_dispatch(sq, PERSISTENT_KERNEL, "SQ_ACTIVE_INST_VALU")
+        busy, _ = _per_dispatch(sq, PERSISTENT_KERNEL, "SQ_BUSY_CYCLES")
+        # SQ_ACTIVE_INST_VALU counts quad-cycles summed over SIMDs; SQ_BUSY_CYCLES is summed over shader engines.
+        d["valu_busy"] = round(4 * active / (SIMDS * busy / SHADER_ENGINES), 4)
+        d["valu_lane_insts_per_cell"] = round(insts * 64 / (T * n ** 3), 2)
+    (out_path or TRAFFIC_FILE).write_text(json.dumps(d, indent=1) + "\n")
+    print(json.dumps(d))
+
+
 def main():
     args = parse()
+    if args.record_profile:
+        record_profile(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the persistent stencil path is single-device "
               "(run with --gpus 1)", file=sys.stderr)

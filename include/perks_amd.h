@@ -14,6 +14,9 @@
  *                                      stencil/3dstencil/j3d.cu:598,612
  *   perks_get_min_width_y              template<class REAL> int getMinWidthY(int,int,int,bool)
  *                                      stencil/3dstencil/common/jacobi_cuda.cuh:12; j3d.cu:69-123
+ *   perks_get_min_width_y_2d           template<class REAL> int getMinWidthY(int,int,int,bool,int,bool)
+ *                                      and getMinWidthY(int,int,bool)
+ *                                      stencil/2dstencil/common/jacobi_cuda.cuh:12-17; jacobi-2d.cu:79-364
  *   perks_jacobi2d_iterative_{f64,f32} template<class REAL> int jacobi_iterative(...)
  *                                      stencil/2dstencil/common/jacobi_cuda.cuh:6-10;
  *                                      stencil/2dstencil/jacobi-2d.cu:369-865
@@ -183,6 +186,17 @@ int perks_j3d_plan(int dtype, int height, int width_y, int width_x, const perks_
 int perks_j3d_min_height(int dtype, int width_y, int width_x, const perks_stencil_opts* opts);
 /* getMinWidthY<REAL>(width_x, width_y, bdimx, isDoubleTile), j3d.cu:69-123. */
 int perks_get_min_width_y(int dtype, int width_x, int width_y, int bdimx, int double_tile);
+/* The 2D getMinWidthY overloads (stencil/2dstencil/common/jacobi_cuda.cuh:12-17;
+ * defined stencil/2dstencil/jacobi-2d.cu:79-364): the number of rows of
+ * width_x cells that the chip's on-chip storage holds, i.e. the height below
+ * which a persistent grid could keep the whole 2D domain resident.
+ * `registers` > 0 counts every CU's VGPR file, use_lds every CU's LDS; as in
+ * the reference, no cache at all (registers == 0 and !use_lds) returns 0.
+ * blkpsm and double_tile do not change the capacity and are accepted for the
+ * signature.  The reference's 3-argument form is registers = 256, use_lds = 1
+ * (its maximum over the cache variants). */
+int perks_get_min_width_y_2d(int dtype, int width_x, int bdimx, int registers, int use_lds, int blkpsm,
+                             int double_tile);
 
 /* ---- 3D Jacobi ------------------------------------------------------------ */
 /* Host buffers, as the reference: h_in[height][width_y][width_x] (x fastest)

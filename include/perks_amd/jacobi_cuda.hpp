@@ -13,6 +13,22 @@
 // reference's coefficients and clamp boundary, run by the persistent kernel:
 //   PERKS_J3D_SHAPE   (enum perks_shape)   PERKS_J3D_METHOD   (enum perks_method)
 //   PERKS_J2D_SHAPE   (enum perks_shape)   PERKS_J2D_METHOD   (enum perks_method)
+//
+// A reference driver builds unmodified against this header when the
+// reference's own declaration header is kept out by its include guard:
+//   g++ -DPERKS_CUDA_HEADER -include perks_amd/jacobi_cuda.hpp j3d.driver.cpp ...
+//
+// Reference arguments that select a CUDA kernel variant are accepted and
+// recorded in perks_stencil_opts, and do not change what runs here:
+//   bdimx         (threads per block, 128/256): the tile shapes are fixed per
+//                 kernel variant for gfx950's 64-lane waves (opts.tune[4]);
+//   useSM         (LDS plane cache on/off): the kernels keep their working
+//                 planes or rows in VGPR rings, LDS only for the row exchange;
+//   isDoubleTile  (16 instead of 8 rows per thread): rows per wave are fixed
+//                 per tile variant;
+//   async         (2D cp.async staging): loads are always asynchronous.
+// blkpsm (blocks per SM) caps the workgroups per CU as in the reference;
+// usewarmup/warmupiteration run untimed warm-up launches.
 #pragma once
 
 #include "../perks_amd.h"
@@ -95,6 +111,23 @@ template <class REAL>
 int getMinWidthY(int width_x, int width_y, int bdimx, bool isDoubleTile = false)
 {
     return perks_get_min_width_y(perks_detail::Abi<REAL>::dtype, width_x, width_y, bdimx, isDoubleTile ? 1 : 0);
+}
+
+// stencil/2dstencil/common/jacobi_cuda.cuh:14 ; jacobi-2d.cu:137-312
+// (registers: the per-thread register budget, 0 = no register cache)
+template <class REAL>
+int getMinWidthY(int width_x, int bdimx, int registers, bool useSM, int blkpsm, bool isDoubleTile)
+{
+    return perks_get_min_width_y_2d(perks_detail::Abi<REAL>::dtype, width_x, bdimx, registers, useSM ? 1 : 0, blkpsm,
+                                    isDoubleTile ? 1 : 0);
+}
+
+// stencil/2dstencil/common/jacobi_cuda.cuh:16 ; jacobi-2d.cu:315-359 (the
+// largest over the reference's cache variants: registers and LDS)
+template <class REAL>
+int getMinWidthY(int width_x, int bdimx, bool isDoubleTile)
+{
+    return perks_get_min_width_y_2d(perks_detail::Abi<REAL>::dtype, width_x, bdimx, 256, 1, 0, isDoubleTile ? 1 : 0);
 }
 
 // stencil/2dstencil/common/jacobi_cuda.cuh:6-10 ; jacobi-2d.cu:369-374

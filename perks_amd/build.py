@@ -1,6 +1,8 @@
 """Builds libperks_amd.so (HIP, gfx950) in-tree.
 
-    python -m perks_amd.build          # compile every HIP source and link
+    python -m perks_amd.build            # compile every HIP source and link
+    python -m perks_amd.build --changed  # recompile only sources whose object is
+                                         # older than the source or a header it includes
 
 Every source under perks_amd/csrc/ (*.hip: device + host, *.cpp: host only) is
 compiled with hipcc for gfx950 and linked into perks_amd/libperks_amd.so.  Flags that matter for results:
@@ -47,9 +49,28 @@ COMMON_FLAGS = [
 ]
 
 
-def _compile(src: pathlib.Path) -> pathlib.Path:
+def _up_to_date(src: pathlib.Path, obj: pathlib.Path) -> bool:
+    """True when obj and its dependency list exist and nothing listed is newer."""
+    dep = obj.with_suffix(".d")
+    if not obj.exists() or not dep.exists():
+        return False
+    t = obj.stat().st_mtime
+    text = dep.read_text().replace("\\\n", " ")
+    files = [f for f in text.split(":", 1)[-1].split() if f]
+    for f in [str(src), *files]:
+        try:
+            if pathlib.Path(f).stat().st_mtime > t:
+                return False
+        except OSError:
+            return False
+    return True
+
+
+def _compile(src: pathlib.Path, changed_only: bool = False) -> pathlib.Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [hipcc(), *COMMON_FLAGS, "-c", str(src), "-o", str(obj)]
+    if changed_only and _up_to_date(src, obj):
+        return obj
+    cmd = [hipcc(), *COMMON_FLAGS, "-MMD", "-MF", str(obj.with_suffix(".d")), "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass does not know the packed-fp32 feature and says so; drop that line.
     noise = "is not a recognized feature for this target"
@@ -61,13 +82,13 @@ def _compile(src: pathlib.Path) -> pathlib.Path:
     return obj
 
 
-def build(verbose: bool = True) -> pathlib.Path:
+def build(verbose: bool = True, changed_only: bool = False) -> pathlib.Path:
     BUILD.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     if not sources:
         raise RuntimeError(f"no HIP sources under {CSRC}")
     with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
-        objs = list(ex.map(_compile, sources))
+        objs = list(ex.map(lambda src: _compile(src, changed_only), sources))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -80,4 +101,4 @@ def build(verbose: bool = True) -> pathlib.Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(changed_only="--changed" in sys.argv[1:])

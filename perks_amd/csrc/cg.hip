@@ -316,38 +316,58 @@ struct CgScratch {
 std::mutex g_cg_mu;
 CgScratch g_cg[64];
 
+// Replaces *slot with a fresh allocation of `bytes`.  The slot is nulled
+// before the allocation, so a failed allocation leaves nothing freed behind.
+int realloc_dev(void** slot, size_t bytes)
+{
+    if (*slot) {
+        void* old = *slot;
+        *slot = nullptr;
+        PERKS_HIP_TRY(hipFree(old));
+    }
+    PERKS_HIP_TRY(hipMalloc(slot, bytes));
+    return PERKS_OK;
+}
+
+#define PERKS_TRY(expr)                 \
+    do {                                \
+        const int perks_rc_ = (expr);   \
+        if (perks_rc_ != PERKS_OK) return perks_rc_; \
+    } while (0)
+
 int ensure(CgScratch& s, size_t n, size_t nval32, int hist_cap)
 {
     if (s.n < n) {
-        for (void* p : {s.p0, s.p1, s.ap, s.r, (void*)s.rf, (void*)s.xf, (void*)s.parts})
-            if (p) PERKS_HIP_TRY(hipFree(p));
-        PERKS_HIP_TRY(hipMalloc(&s.p0, n * 8));
-        PERKS_HIP_TRY(hipMalloc(&s.p1, n * 8));
-        PERKS_HIP_TRY(hipMalloc(&s.ap, n * 8));
-        PERKS_HIP_TRY(hipMalloc(&s.r, n * 8));
-        PERKS_HIP_TRY(hipMalloc(&s.rf, n * 4));
-        PERKS_HIP_TRY(hipMalloc(&s.xf, n * 4));
+        // The sizes are recorded only once every buffer is in place.
+        s.n = 0;
+        s.nseg = 0;
         const int nseg = (int)((n + kSeg - 1) / kSeg);
-        PERKS_HIP_TRY(hipMalloc(&s.parts, sizeof(double) * 2 * (size_t)nseg));
+        PERKS_TRY(realloc_dev(&s.p0, n * 8));
+        PERKS_TRY(realloc_dev(&s.p1, n * 8));
+        PERKS_TRY(realloc_dev(&s.ap, n * 8));
+        PERKS_TRY(realloc_dev(&s.r, n * 8));
+        PERKS_TRY(realloc_dev((void**)&s.rf, n * 4));
+        PERKS_TRY(realloc_dev((void**)&s.xf, n * 4));
+        PERKS_TRY(realloc_dev((void**)&s.parts, sizeof(double) * 2 * (size_t)nseg));
         s.n = n;
         s.nseg = nseg;
     }
     if (nval32 > s.nvalf) {
-        if (s.valf) PERKS_HIP_TRY(hipFree(s.valf));
-        PERKS_HIP_TRY(hipMalloc(&s.valf, nval32 * 4));
+        s.nvalf = 0;
+        PERKS_TRY(realloc_dev((void**)&s.valf, nval32 * 4));
         s.nvalf = nval32;
     }
     if (!s.small) PERKS_HIP_TRY(hipMalloc(&s.small, 64));
     if (s.hist_cap < hist_cap) {
-        if (s.hist) PERKS_HIP_TRY(hipFree(s.hist));
-        PERKS_HIP_TRY(hipMalloc(&s.hist, sizeof(double) * (size_t)hist_cap));
+        s.hist_cap = 0;
+        PERKS_TRY(realloc_dev((void**)&s.hist, sizeof(double) * (size_t)hist_cap));
         s.hist_cap = hist_cap;
     }
     return PERKS_OK;
 }
 
 template <typename V, int FMT>
-int launch_cg(CgArgs<V>& a, hipStream_t stream, int blkpsm, int* grid_out)
+int launch_cg(CgArgs<V>& a, GridSync* gs, hipStream_t stream, int blkpsm, int* grid_out)
 {
     const void* fn = reinterpret_cast<const void*>(&cg_persistent_kernel<V, FMT>);
     int occ = 0;
@@ -362,8 +382,7 @@ int launch_cg(CgArgs<V>& a, hipStream_t stream, int blkpsm, int* grid_out)
     if (rc != PERKS_OK) return rc;
     int G = caps.cu_count * occ;
     if (G > a.nseg) G = a.nseg > 0 ? a.nseg : 1;
-    rc = grid_sync_get(kSyncCg, &a.sync);
-    if (rc != PERKS_OK) return rc;
+    a.sync = gs;
     a.timeout = grid_sync_timeout();
     void* args[] = {&a};
     PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(G), dim3(kLanes), args, 0, stream));
@@ -372,11 +391,11 @@ int launch_cg(CgArgs<V>& a, hipStream_t stream, int blkpsm, int* grid_out)
 }
 
 template <typename V>
-int launch_cg_fmt(int fmt, CgArgs<V>& a, hipStream_t stream, int blkpsm, int* grid_out)
+int launch_cg_fmt(int fmt, CgArgs<V>& a, GridSync* gs, hipStream_t stream, int blkpsm, int* grid_out)
 {
-    if (fmt == FMT_CSR) return launch_cg<V, FMT_CSR>(a, stream, blkpsm, grid_out);
-    if (a.ndiag == 7) return launch_cg<V, FMT_DIA7>(a, stream, blkpsm, grid_out);
-    return launch_cg<V, FMT_DIA>(a, stream, blkpsm, grid_out);
+    if (fmt == FMT_CSR) return launch_cg<V, FMT_CSR>(a, gs, stream, blkpsm, grid_out);
+    if (a.ndiag == 7) return launch_cg<V, FMT_DIA7>(a, gs, stream, blkpsm, grid_out);
+    return launch_cg<V, FMT_DIA>(a, gs, stream, blkpsm, grid_out);
 }
 
 }  // namespace
@@ -400,6 +419,9 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
     CgScratch& S = g_cg[dev];
     const int hist_cap = std::max(1, o->max_iter + 1);
     int rc = ensure(S, (size_t)n, refine ? nval : 0, hist_cap);
+    if (rc != PERKS_OK) return rc;
+    SyncLease lease;
+    rc = lease.acquire();
     if (rc != PERKS_OK) return rc;
 
     hipEvent_t ev0, ev1;
@@ -447,7 +469,7 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
         a.hist_cap = hist_cap;
         a.x_is_zero = o->x0_is_zero;
         a.static_iter = o->static_iter;
-        rc = launch_cg_fmt<double>(fmt, a, stream, o->blkpsm, &grid);
+        rc = launch_cg_fmt<double>(fmt, a, lease.gs, stream, o->blkpsm, &grid);
         if (rc != PERKS_OK) return rc;
     } else {
         // FP32 copy of the matrix values, once per solve.
@@ -461,9 +483,7 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
     PERKS_HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0.f;
     PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    GridSync* gs = nullptr;
-    rc = grid_sync_get(kSyncCg, &gs);
-    if (rc != PERKS_OK) return rc;
+    GridSync* gs = lease.gs;
     if (!refine) {
         rc = grid_sync_check(gs, stream);
         if (rc != PERKS_OK) return rc;
@@ -503,7 +523,7 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
             a.hist = nullptr;
             a.x_is_zero = (outer == 0) ? o->x0_is_zero : 0;
             PERKS_HIP_TRY(hipEventRecord(ev0, stream));
-            rc = launch_cg_fmt<double>(fmt, a, stream, o->blkpsm, &grid);
+            rc = launch_cg_fmt<double>(fmt, a, lease.gs, stream, o->blkpsm, &grid);
             if (rc != PERKS_OK) return rc;
             double small[3];
             PERKS_HIP_TRY(hipMemcpyAsync(small, S.small, sizeof(small), hipMemcpyDeviceToHost, stream));
@@ -557,7 +577,7 @@ int cg_solve(int fmt, const perks_csr* csr, const perks_dia* dia, const double* 
             f.bb_out = S.small + 1;
             f.iters_out = (int*)(S.small + 2);
             f.x_is_zero = 1;
-            rc = launch_cg_fmt<float>(fmt, f, stream, o->blkpsm, &grid);
+            rc = launch_cg_fmt<float>(fmt, f, lease.gs, stream, o->blkpsm, &grid);
             if (rc != PERKS_OK) return rc;
             // (3) x += d
             hipLaunchKernelGGL(add_f32_to_f64, dim3(nb), dim3(256), 0, stream, d_x, S.xf, n);
@@ -609,6 +629,9 @@ int cg_solve_f32(int fmt, const perks_csr_f32* csr, const perks_dia_f32* dia, co
     const int hist_cap = std::max(1, o->max_iter + 1);
     int rc = ensure(S, (size_t)n, 0, hist_cap);
     if (rc != PERKS_OK) return rc;
+    SyncLease lease;
+    rc = lease.acquire();
+    if (rc != PERKS_OK) return rc;
     CgArgs<float> f;
     std::memset(&f, 0, sizeof(f));
     f.n = n;
@@ -646,7 +669,7 @@ int cg_solve_f32(int fmt, const perks_csr_f32* csr, const perks_dia_f32* dia, co
     PERKS_HIP_TRY(hipEventCreate(&ev1));
     PERKS_HIP_TRY(hipEventRecord(ev0, stream));
     int grid = 0;
-    rc = launch_cg_fmt<float>(fmt, f, stream, o->blkpsm, &grid);
+    rc = launch_cg_fmt<float>(fmt, f, lease.gs, stream, o->blkpsm, &grid);
     if (rc != PERKS_OK) return rc;
     PERKS_HIP_TRY(hipEventRecord(ev1, stream));
     PERKS_HIP_TRY(hipEventSynchronize(ev1));
@@ -654,10 +677,7 @@ int cg_solve_f32(int fmt, const perks_csr_f32* csr, const perks_dia_f32* dia, co
     PERKS_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
-    GridSync* gs = nullptr;
-    rc = grid_sync_get(kSyncCg, &gs);
-    if (rc != PERKS_OK) return rc;
-    rc = grid_sync_check(gs, stream);
+    rc = grid_sync_check(lease.gs, stream);
     if (rc != PERKS_OK) return rc;
     double small[3];
     PERKS_HIP_TRY(hipMemcpy(small, S.small, sizeof(small), hipMemcpyDeviceToHost));

@@ -130,16 +130,42 @@ __device__ __forceinline__ bool grid_sync(GridSync* gs, unsigned int nblocks, un
 constexpr int kCoopMaxWavesPerCu = 16;
 
 // ---- host side ----------------------------------------------------------------
-enum GridSyncUser { kSyncJ3d = 0, kSyncJ2d = 1, kSyncCg = 2, kSyncUsers = 3 };
+// Barrier blocks come from a per-device pool.  A launch leases a block of its
+// own for the time from the launch until grid_sync_check has seen the launch
+// complete, so launches from different host threads or streams never share
+// counters or abort words.  A batch plan owns a block for its whole lifetime
+// (grid_sync_alloc / grid_sync_free), so perks_release() cannot free a block
+// that a captured graph still names.
 
-// The per-device block of `user` (allocated and zeroed on first use, kept
-// until grid_sync_release).
-int grid_sync_get(int user, GridSync** out);
-// Wait limit in wall-clock ticks for one barrier (10 s).
+// Takes a zeroed block from the current device's pool (allocating one when
+// the pool is empty).
+int grid_sync_acquire(GridSync** out);
+// Returns a block taken with grid_sync_acquire to the pool of the device it
+// came from.  The block must be idle (its launch completed and checked).
+void grid_sync_return(GridSync* gs);
+// A block owned by the caller (allocated and zeroed on the current device).
+int grid_sync_alloc(GridSync** out);
+int grid_sync_free(GridSync* gs);
+// Wait limit in wall-clock ticks for one barrier (10 s of the current device).
 unsigned long long grid_sync_timeout();
 // After a launch has completed: PERKS_OK, or PERKS_ERR_BARRIER_TIMEOUT (the
 // block is then cleared for the next launch).  Synchronises `stream`.
 int grid_sync_check(GridSync* gs, hipStream_t stream);
+// Frees the current device's idle pooled blocks (leased and plan-owned blocks
+// stay valid).
 int grid_sync_release();
+
+// Scope guard for a leased block.
+struct SyncLease {
+    GridSync* gs = nullptr;
+    SyncLease() = default;
+    SyncLease(const SyncLease&) = delete;
+    SyncLease& operator=(const SyncLease&) = delete;
+    int acquire() { return gs ? 0 : grid_sync_acquire(&gs); }
+    ~SyncLease()
+    {
+        if (gs) grid_sync_return(gs);
+    }
+};
 
 }  // namespace perks

@@ -400,9 +400,10 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     std::lock_guard<std::mutex> lk(g_mu2);
     rc = g_work[dev].upload(w, stream);
     if (rc != PERKS_OK) return rc;
-    GridSync* gs = nullptr;
-    rc = grid_sync_get(kSyncJ2d, &gs);
+    SyncLease lease;
+    rc = lease.acquire();
     if (rc != PERKS_OK) return rc;
+    GridSync* gs = lease.gs;
     Launch l;
     rc = prepare_launch(dtype, o, cf, c, w, g_work[dev], gs, &l);
     if (rc != PERKS_OK) return rc;
@@ -447,7 +448,8 @@ struct perks_batch_plan {
     perks::Work work;
     perks::DeviceWork dw;
     perks::Launch launch;
-    perks::GridSync* gs = nullptr;
+    perks::GridSync* gs = nullptr;  // the plan's own barrier block (grid_sync_alloc)
+    std::mutex run_mu;              // runs of one plan share that block: one at a time
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int tuned = 0;
@@ -520,7 +522,7 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
     p->dtype = dtype;
     p->opts = *o;
     PERKS_HIP_TRY(hipGetDevice(&p->device));
-    rc = grid_sync_get(kSyncJ2d, &p->gs);
+    rc = grid_sync_alloc(&p->gs);
     if (rc != PERKS_OK) {
         delete p;
         return rc;
@@ -528,6 +530,7 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
     int tile = 0;
     rc = resolve_tile(dtype, o, specs.data(), ngrids, &tile);
     if (rc != PERKS_OK) {
+        (void)grid_sync_free(p->gs);
         delete p;
         return rc;
     }
@@ -609,6 +612,7 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
         if (p->exec) (void)hipGraphExecDestroy(p->exec);
         if (p->graph) (void)hipGraphDestroy(p->graph);
         (void)p->dw.release();
+        (void)grid_sync_free(p->gs);
         delete p;
         return rc;
     }
@@ -637,6 +641,7 @@ int perks_jacobi2d_batch_run(perks_batch_plan* p, void* stream_v, perks_run_repo
     int dev = 0;
     PERKS_HIP_TRY(hipGetDevice(&dev));
     if (dev != p->device) return PERKS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(p->run_mu);
     hipEvent_t ev0, ev1;
     PERKS_HIP_TRY(hipEventCreate(&ev0));
     PERKS_HIP_TRY(hipEventCreate(&ev1));
@@ -680,6 +685,7 @@ void perks_jacobi2d_batch_plan_destroy(perks_batch_plan* p)
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     (void)p->dw.release();
+    (void)perks::grid_sync_free(p->gs);
     delete p;
 }
 

@@ -225,9 +225,10 @@ int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, 
     a.sr = (dtype == PERKS_F16 || dtype == PERKS_BF16) && o->rounding == PERKS_ROUND_STOCHASTIC;
     a.seed = o->round_seed;
     KCoefs kc = cf;
-    GridSync* gs = nullptr;
-    rc = grid_sync_get(kSyncJ3d, &gs);
+    SyncLease lease;
+    rc = lease.acquire();
     if (rc != PERKS_OK) return rc;
+    GridSync* gs = lease.gs;
     a.sync = gs;
     a.timeout = grid_sync_timeout();
 

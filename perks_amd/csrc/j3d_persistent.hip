@@ -225,9 +225,10 @@ int run_persistent_fmt(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, 
     rc = plan_persistent(dtype, nz, ny, nx, o, &caps, occ, &pl);
     if (rc != PERKS_OK) return rc;
 
-    GridSync* gs = nullptr;
-    rc = grid_sync_get(kSyncJ3d, &gs);
+    SyncLease lease;
+    rc = lease.acquire();
     if (rc != PERKS_OK) return rc;
+    GridSync* gs = lease.gs;
 
     PersistArgs a;
     std::memset(&a, 0, sizeof(a));

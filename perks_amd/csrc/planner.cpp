@@ -39,6 +39,17 @@ int plan_plain(int nz, int ny, int nx, int cu_count, perks_plan_info* out)
     return PERKS_OK;
 }
 
+int min_rows_2d_caps(int dtype, int nx, bool use_regs, bool use_lds, const perks_device_caps* caps)
+{
+    if (nx <= 0 || !caps) return PERKS_ERR_INVALID_ARG;
+    const long long esz = dtype == PERKS_F64 ? 8 : (dtype == PERKS_F32 ? 4 : 2);
+    const long long per_cu = (use_lds ? (long long)caps->lds_per_cu : 0) +
+                             (use_regs ? (long long)caps->vgprs_per_simd * caps->simds_per_cu * caps->warp_size * 4LL
+                                       : 0);
+    const long long rows = (long long)caps->cu_count * per_cu / (nx * esz);
+    return (int)(rows < 0x7fffffffLL ? rows : 0x7fffffffLL);
+}
+
 }  // namespace perks
 
 extern "C" int perks_j3d_plan(int dtype, int height, int width_y, int width_x, const perks_stencil_opts* opts,
@@ -65,6 +76,21 @@ extern "C" int perks_j3d_min_height(int dtype, int width_y, int width_x, const p
 {
     if (!opts || width_y <= 0 || width_x <= 0) return PERKS_ERR_INVALID_ARG;
     return perks::min_height_persistent(dtype, width_y, width_x, opts);
+}
+
+extern "C" int perks_get_min_width_y_2d(int dtype, int width_x, int bdimx, int registers, int use_lds, int blkpsm,
+                                        int double_tile)
+{
+    (void)bdimx;
+    (void)blkpsm;
+    (void)double_tile;
+    if (width_x <= 0 || registers < 0) return PERKS_ERR_INVALID_ARG;
+    if (dtype != PERKS_F64 && dtype != PERKS_F32 && dtype != PERKS_F16 && dtype != PERKS_BF16)
+        return PERKS_ERR_UNSUPPORTED;
+    perks_device_caps caps;
+    const int rc = perks::device_caps_cached(&caps);
+    if (rc != PERKS_OK) return rc;
+    return perks::min_rows_2d_caps(dtype, width_x, registers > 0, use_lds != 0, &caps);
 }
 
 extern "C" int perks_get_min_width_y(int dtype, int width_x, int width_y, int bdimx, int double_tile)

@@ -163,13 +163,15 @@ struct RandSeq {
 template <typename T>
 void gen_tridiag(int32_t n, int32_t* I, int32_t* J, T* val)
 {
+    // rand() / RAND_MAX with T operands: RAND_MAX converts to T, as the
+    // reference's implicit conversion does (2^31 - 1 rounds to 2^31 in float).
     RandSeq rs;
     const int32_t nz = (n - 2) * 3 + 4;
     I[0] = 0;
     J[0] = 0;
     J[1] = 1;
-    val[0] = static_cast<T>(rs.next()) / RAND_MAX + 10.0f;
-    val[1] = static_cast<T>(rs.next()) / RAND_MAX;
+    val[0] = static_cast<T>(rs.next()) / static_cast<T>(RAND_MAX) + 10.0f;
+    val[1] = static_cast<T>(rs.next()) / static_cast<T>(RAND_MAX);
     for (int32_t i = 1; i < n; ++i) {
         I[i] = i > 1 ? I[i - 1] + 3 : 2;
         const int32_t start = (i - 1) * 3 + 2;
@@ -177,8 +179,8 @@ void gen_tridiag(int32_t n, int32_t* I, int32_t* J, T* val)
         J[start + 1] = i;
         if (i < n - 1) J[start + 2] = i + 1;
         val[start] = val[start - 1];
-        val[start + 1] = static_cast<T>(rs.next()) / RAND_MAX + 10.0f;
-        if (i < n - 1) val[start + 2] = static_cast<T>(rs.next()) / RAND_MAX;
+        val[start + 1] = static_cast<T>(rs.next()) / static_cast<T>(RAND_MAX) + 10.0f;
+        if (i < n - 1) val[start + 2] = static_cast<T>(rs.next()) / static_cast<T>(RAND_MAX);
     }
     I[n] = nz;
 }

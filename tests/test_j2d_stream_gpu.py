@@ -158,6 +158,35 @@ def test_stream_batch_plan_graph_replays(gpu):
     plan.close()
 
 
+def test_stream_batch_plan_survives_release(gpu):
+    """A plan owns its barrier block: perks_release() between planning and
+    running frees the library's pooled scratch but leaves the plan valid."""
+    torch = gpu
+    specs, fields, bufs = _batch(torch, "f32", 6, 23)
+    say(f"[j2s] plan -> release -> run {specs}")
+    si = P.shape_info("2d5pt")
+    coefs = [0.2] * 5
+    plan = P.BatchPlan([(a, b, wy, wx, st) for (a, b), (wy, wx, st) in zip(bufs, specs)], dtype="f32",
+                       autotune=False, shape="2d5pt", method="persistent", boundary="zero", coefs=coefs,
+                       tune=tune())
+    # a direct persistent run leases a pooled block, which release then frees
+    f0 = fields[0]
+    P.jacobi2d_run(f0, 3, shape="2d5pt", method="persistent", boundary="zero", coefs=coefs)
+    P.release()
+    wants = [O.iterate(f, O.stencil_from_shape(si, "zero", coefs=coefs), st) for f, (_, _, st) in zip(fields, specs)]
+    for replay in range(2):
+        for (a, b), f in zip(bufs, fields):
+            a.copy_(torch.from_numpy(f.copy()).view(-1).cuda())
+            b.fill_(-1.0)
+        torch.cuda.synchronize()
+        plan.run()
+        for i, ((a, b), (wy, wx, st)) in enumerate(zip(bufs, specs)):
+            got = (b if plan.result_in_b[i] else a).cpu().numpy().reshape(wy, wx)
+            check_equal(got, wants[i], f"after release, replay {replay} grid {i} {wy}x{wx} steps {st}")
+        P.release()
+    plan.close()
+
+
 # ---- two columns per lane (tile variant 4; even widths, FP32/FP16/BF16) -------------
 PAIR = 4
 
