@@ -298,8 +298,13 @@ int perks_jacobi2d_batch(int dtype, perks_grid2d* grids, int ngrids, const perks
  * the fastest; then it captures the launch in a hipGraph.  Each
  * perks_jacobi2d_batch_run replays the graph on `stream` and waits: every grid
  * runs its steps from what its buffer a holds, as perks_jacobi2d_batch does.
- * The buffers, sizes and step counts are fixed by the plan.  Two plans (or a
- * plan and a direct batch call) must not run at the same time on one device.
+ * The buffers, sizes and step counts are fixed by the plan.  A plan owns its
+ * grid-barrier block (perks_release() leaves it valid); runs of one plan are
+ * serialised.  The captured launch is an ordinary (not cooperative) launch of
+ * at most CUs x occupancy workgroups whose barrier needs every workgroup
+ * resident at once: a replay needs the device to itself, and other work that
+ * holds CUs meanwhile makes its barrier wait time out (PERKS_ERR_BARRIER_TIMEOUT
+ * after 10 s) instead of completing.
  * plan_info: result_in_b[i] per grid (ngrids entries, may be NULL) and
  * config[6] = {tile variant, steps per pass, workgroups per CU, workgroups,
  * autotuned, passes} (may be NULL). */

@@ -385,7 +385,7 @@ int launch_cg(CgArgs<V>& a, GridSync* gs, hipStream_t stream, int blkpsm, int* g
     a.sync = gs;
     a.timeout = grid_sync_timeout();
     void* args[] = {&a};
-    PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(G), dim3(kLanes), args, 0, stream));
+    PERKS_HIP_TRY(launch_persistent(fn, dim3(G), dim3(kLanes), args, 0, stream));
     if (grid_out) *grid_out = G;
     return PERKS_OK;
 }

@@ -155,6 +155,15 @@ int grid_sync_check(GridSync* gs, hipStream_t stream);
 // stay valid).
 int grid_sync_release();
 
+// Launches a persistent grid (at most CUs x measured occupancy workgroups)
+// cooperatively.  The HIP runtime refuses a cooperative launch with
+// hipErrorIllegalState while another process on the device holds its
+// cooperative queue; the same grid then goes out as an ordinary launch, which
+// an otherwise idle device places whole, and whose barrier's bounded wait
+// turns a grid that is not co-resident into PERKS_ERR_BARRIER_TIMEOUT rather
+// than a hang.
+hipError_t launch_persistent(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t stream);
+
 // Scope guard for a leased block.
 struct SyncLease {
     GridSync* gs = nullptr;

@@ -104,6 +104,19 @@ int grid_sync_check(GridSync* gs, hipStream_t stream)
     return PERKS_ERR_BARRIER_TIMEOUT;
 }
 
+hipError_t launch_persistent(const void* fn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t stream)
+{
+    hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream);
+    if (e != hipErrorIllegalState) return e;
+    (void)hipGetLastError();  // clear the refusal
+    static std::once_flag note;
+    std::call_once(note, [] {
+        std::fprintf(stderr, "perks_amd: the cooperative queue is held by another process; persistent grids run as "
+                             "ordinary launches of the same size\n");
+    });
+    return hipLaunchKernel(fn, grid, block, args, lds, stream);
+}
+
 int grid_sync_release()
 {
     int dev = 0;

@@ -322,7 +322,7 @@ int enqueue(Launch& l, const perks_stencil_opts* o, const Work& w, hipStream_t s
         return PERKS_OK;
     }
     void* args[] = {&l.args, &l.cf};
-    if (cooperative) PERKS_HIP_TRY(hipLaunchCooperativeKernel(l.fn, dim3(l.grid), dim3(256), args, 0, stream));
+    if (cooperative) PERKS_HIP_TRY(launch_persistent(l.fn, dim3(l.grid), dim3(256), args, 0, stream));
     else PERKS_HIP_TRY(hipLaunchKernel(l.fn, dim3(l.grid), dim3(256), args, 0, stream));
     return PERKS_OK;
 }

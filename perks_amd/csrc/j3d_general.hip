@@ -266,7 +266,7 @@ int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, 
         a.step_offset = 0;
         void* args[] = {&a, &kc};
         if (steps > 0)
-            PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, (unsigned)lds, stream));
+            PERKS_HIP_TRY(launch_persistent(fn, dim3(grid), dim3(threads), args, lds, stream));
     }
     PERKS_HIP_TRY(hipEventRecord(ev1, stream));
     PERKS_HIP_TRY(hipEventSynchronize(ev1));

@@ -249,7 +249,7 @@ int run_persistent_fmt(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, 
     PERKS_HIP_TRY(hipEventCreate(&ev1));
     PERKS_HIP_TRY(hipEventRecord(ev0, stream));
     if (steps > 0)
-        PERKS_HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(pl.grid[0]), dim3(pl.block), args, 0, stream));
+        PERKS_HIP_TRY(launch_persistent(fn, dim3(pl.grid[0]), dim3(pl.block), args, 0, stream));
     PERKS_HIP_TRY(hipEventRecord(ev1, stream));
     PERKS_HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0.f;
