@@ -290,6 +290,22 @@ typedef struct perks_grid2d {
 int perks_jacobi2d_batch(int dtype, perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts, void* stream,
                          perks_run_report* report);
 
+/* perks_jacobi2d_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  The launch's scratch --
+ * grid-barrier block, work list, per-pass active counts -- lives in a fixed
+ * device array (about 0.9 MiB) that each GPU holds until the process ends,
+ * so a batch may hold at most PERKS_NOALLOC_MAX_GRIDS grids with steps and
+ * PERKS_NOALLOC_MAX_PASSES passes (per-step method: time steps) of its
+ * longest grid; larger batches return PERKS_ERR_UNSUPPORTED_SIZE.  Calls on
+ * one GPU share that array: calls from one process are serialised, and the
+ * call waits for its work before returning.  report->elapsed_ms is 0 (no
+ * timing events are created).  Results equal perks_jacobi2d_batch's. */
+#define PERKS_NOALLOC_MAX_GRIDS 4096
+#define PERKS_NOALLOC_MAX_PASSES 16384
+int perks_jacobi2d_batch_noalloc(int dtype, perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts,
+                                 void* stream, perks_run_report* report);
+
 /* A reusable plan for a batch (no reference counterpart: the batched form is
  * an addition of this library).  Creating it uploads the work list once;
  * with autotune = 1 it times candidate launch configurations (tile shape,

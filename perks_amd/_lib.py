@@ -149,6 +149,8 @@ SIGNATURES = {
     "perks_j3d_min_height": (_I, [_I, _I, _I, C.POINTER(StencilOpts)]),
     "perks_get_min_width_y": (_I, [_I, _I, _I, _I, _I]),
     "perks_get_min_width_y_2d": (_I, [_I, _I, _I, _I, _I, _I, _I]),
+    "perks_jacobi2d_batch_noalloc": (_I, [_I, C.POINTER(Grid2d), _I, C.POINTER(StencilOpts), _P,
+                                          C.POINTER(RunReport)]),
     "perks_j3d_iterative_f64": (_I, [_P, _I, _I, _I, _P, _I, C.POINTER(StencilOpts), C.POINTER(RunReport)]),
     "perks_j3d_iterative_f32": (_I, [_P, _I, _I, _I, _P, _I, C.POINTER(StencilOpts), C.POINTER(RunReport)]),
     "perks_j3d_device_f64": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(StencilOpts), _P, C.POINTER(RunReport)]),

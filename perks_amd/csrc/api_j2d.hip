@@ -116,4 +116,25 @@ int perks_jacobi2d_batch(int dtype, perks_grid2d* grids, int ngrids, const perks
     return rc;
 }
 
+int perks_jacobi2d_batch_noalloc(int dtype, perks_grid2d* grids, int ngrids, const perks_stencil_opts* opts,
+                                 void* stream, perks_run_report* report)
+{
+    if (!grids || ngrids <= 0 || !opts) return PERKS_ERR_INVALID_ARG;
+    if (dtype < PERKS_F64 || dtype > PERKS_BF16) return PERKS_ERR_INVALID_ARG;
+    if (ngrids > PERKS_NOALLOC_MAX_GRIDS) {
+        std::fprintf(stderr, "perks_amd: perks_jacobi2d_batch_noalloc: %d grids exceed its fixed scratch (at most %d)\n",
+                     ngrids, PERKS_NOALLOC_MAX_GRIDS);
+        return PERKS_ERR_UNSUPPORTED_SIZE;
+    }
+    std::vector<perks::J2dSpec> specs((size_t)ngrids);
+    for (int i = 0; i < ngrids; ++i) {
+        const int rc = perks::validate2(opts, grids[i].width_y, grids[i].width_x, grids[i].steps);
+        if (rc != PERKS_OK) return rc;
+        specs[i] = perks::J2dSpec{grids[i].a, grids[i].b, grids[i].width_y, grids[i].width_x, grids[i].steps, 0};
+    }
+    const int rc = perks::j2d_run_noalloc(dtype, specs.data(), ngrids, opts, (hipStream_t)stream, report);
+    for (int i = 0; i < ngrids; ++i) grids[i].result_in_b = specs[i].result_in_b;
+    return rc;
+}
+
 }  // extern "C"

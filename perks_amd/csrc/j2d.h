@@ -42,6 +42,9 @@ struct J2dArgs {
 };
 
 int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep);
+// j2d_run with the launch's scratch in a fixed device array: no device allocation.
+int j2d_run_noalloc(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream,
+                    perks_run_report* rep);
 int j2d_pass_levels(int dtype, int shape, const perks_stencil_opts* o, int tile);
 int release_j2d_scratch();
 

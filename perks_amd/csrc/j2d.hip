@@ -428,6 +428,114 @@ int j2d_run(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipSt
     return PERKS_OK;
 }
 
+// ---- the batch without device allocation -----------------------------------
+// Callers that cannot allocate device memory (tight serving loops, harnesses
+// that forbid hipMalloc) run a batch through perks_jacobi2d_batch_noalloc.
+// Everything the launch needs beyond the caller's fields -- the grid barrier
+// block, the work list and the per-pass count of active grids -- lives in
+// `scratch`, a file-scope device array of fixed size that each GPU holds for
+// the life of the process.  In host code the array's name is only a symbol:
+// its device address on the current GPU comes from hipGetSymbolAddress (once
+// per device) and only that address is passed to kernels and copies.
+namespace {
+
+constexpr size_t kNoAllocSyncOff = 0;
+constexpr size_t kNoAllocGridsOff = (sizeof(GridSync) + 255) / 256 * 256;
+constexpr size_t kNoAllocActiveOff =
+    kNoAllocGridsOff + ((size_t)PERKS_NOALLOC_MAX_GRIDS * sizeof(J2dGrid) + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = kNoAllocActiveOff + (size_t)PERKS_NOALLOC_MAX_PASSES * sizeof(int);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_noalloc_mu;                 // one batch at a time per process and device
+unsigned char* g_noalloc_ptr[64] = {};   // device address of `scratch` per device
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device).  g_noalloc_mu held.
+int noalloc_scratch(int dev, unsigned char** out)
+{
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "perks_amd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return PERKS_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return PERKS_OK;
+}
+
+}  // namespace
+
+int j2d_run_noalloc(int dtype, J2dSpec* specs, int n, const perks_stencil_opts* o, hipStream_t stream,
+                    perks_run_report* rep)
+{
+    if (!o || !specs || n <= 0) return PERKS_ERR_INVALID_ARG;
+    const ShapeDesc* sd = nullptr;
+    int rc = check_opts(o, &sd);
+    if (rc != PERKS_OK) return rc;
+    KCoefs cf;
+    rc = resolve_coefs(o, &cf, nullptr);
+    if (rc != PERKS_OK) return rc;
+    int tile = 0;
+    rc = resolve_tile(dtype, o, specs, n, &tile);
+    if (rc != PERKS_OK) return rc;
+    const Config c{tile, j2d_pass_levels(dtype, o->shape, o, tile), o->tune[3]};
+    int tx, ty;
+    tile_dims(dtype, c.tile, &tx, &ty);
+    Work w;
+    rc = build_work(specs, n, sd->radius, tx, ty, c.levels, &w);
+    if (rc != PERKS_OK) return rc;
+    if (w.grids.size() > (size_t)PERKS_NOALLOC_MAX_GRIDS || w.active.size() > (size_t)PERKS_NOALLOC_MAX_PASSES) {
+        std::fprintf(stderr,
+                     "perks_amd: perks_jacobi2d_batch_noalloc: %zu grids and %zu passes exceed its fixed scratch "
+                     "(at most %d grids and %d passes)\n",
+                     w.grids.size(), w.active.size(), PERKS_NOALLOC_MAX_GRIDS, PERKS_NOALLOC_MAX_PASSES);
+        return PERKS_ERR_UNSUPPORTED_SIZE;
+    }
+    for (int i = 0; i < n; ++i) specs[i].result_in_b = w.result_in_b[i];
+    if (rep) std::memset(rep, 0, sizeof(*rep));
+    if (w.grids.empty()) return PERKS_OK;
+    int dev = 0;
+    PERKS_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return PERKS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    unsigned char* base = nullptr;
+    rc = noalloc_scratch(dev, &base);
+    if (rc != PERKS_OK) return rc;
+    GridSync* gs = reinterpret_cast<GridSync*>(base + kNoAllocSyncOff);
+    DeviceWork dw;  // a view of the scratch: never released
+    dw.grids = reinterpret_cast<J2dGrid*>(base + kNoAllocGridsOff);
+    dw.active = reinterpret_cast<int*>(base + kNoAllocActiveOff);
+    // The array keeps the last call's contents: the barrier block must start
+    // at zero, the work list is rewritten whole.
+    PERKS_HIP_TRY(hipMemsetAsync(gs, 0, sizeof(GridSync), stream));
+    PERKS_HIP_TRY(hipMemcpyAsync(dw.grids, w.grids.data(), sizeof(J2dGrid) * w.grids.size(), hipMemcpyHostToDevice,
+                                 stream));
+    PERKS_HIP_TRY(hipMemcpyAsync(dw.active, w.active.data(), sizeof(int) * w.active.size(), hipMemcpyHostToDevice,
+                                 stream));
+    Launch l;
+    rc = prepare_launch(dtype, o, cf, c, w, dw, gs, &l);
+    if (rc != PERKS_OK) return rc;
+    rc = enqueue(l, o, w, stream, true);
+    if (rc != PERKS_OK) return rc;
+    // Wait before the scratch can be rewritten by the next call.
+    if (o->method != PERKS_METHOD_PLAIN) {
+        rc = grid_sync_check(gs, stream);
+        if (rc != PERKS_OK) return rc;
+    } else {
+        PERKS_HIP_TRY(hipStreamSynchronize(stream));
+    }
+    fill_report(rep, 0.f, w, l, c, dtype);
+    return PERKS_OK;
+}
+
 int release_j2d_scratch()
 {
     int dev = 0;
