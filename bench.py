@@ -695,8 +695,8 @@ def run_cg(args):
 
 # ---------------------------------------------------------------------------- recorded counter profile
 def _counter_rows(directory):
-    """(kernel name, dispatch id, counter, value) of every rocprofv3 counter
-    CSV under `directory`."""
+    """(kernel name, dispatch id, counter, value, grid size) of every
+    rocprofv3 counter CSV under `directory`."""
     import csv
     out = []
     files = sorted(pathlib.Path(directory).rglob("*counter_collection.csv"))
@@ -706,17 +706,21 @@ def _counter_rows(directory):
         with open(f, newline="") as fh:
             for row in csv.DictReader(fh):
                 out.append((row["Kernel_Name"], (str(f), row.get("Dispatch_Id", "")), row["Counter_Name"],
-                            float(row["Counter_Value"])))
+                            float(row["Counter_Value"]), int(row.get("Grid_Size") or 0)))
     return out
 
 
-def _per_dispatch(rows, kernel_substr, counter):
+def _per_dispatch(rows, kernel_substr, counter, largest_grid=False):
     """Average over the dispatches of a kernel of the counter's value (summed
-    over the rows of one dispatch)."""
+    over the rows of one dispatch); with largest_grid, only the dispatches of
+    the kernel's largest grid (the field-sized copies, not small ones)."""
     acc = {}
-    for name, disp, cname, v in rows:
-        if kernel_substr in name and cname == counter:
-            acc[disp] = acc.get(disp, 0.0) + v
+    sel = [r for r in rows if kernel_substr in r[0] and r[2] == counter]
+    if largest_grid and sel:
+        g = max(r[4] for r in sel)
+        sel = [r for r in sel if r[4] == g]
+    for name, disp, cname, v, _ in sel:
+        acc[disp] = acc.get(disp, 0.0) + v
     if not acc:
         raise SystemExit(f"no {counter} rows for a kernel matching {kernel_substr!r}")
     return sum(acc.values()) / len(acc), len(acc)
@@ -736,11 +740,11 @@ def record_profile(args, out_path=None):
     fetch_rows, write_rows = _counter_rows(dirs[0]), _counter_rows(dirs[1])
     fetch, nd = _per_dispatch(fetch_rows, PERSISTENT_KERNEL, "FETCH_SIZE")
     write, _ = _per_dispatch(write_rows, PERSISTENT_KERNEL, "WRITE_SIZE")
-    cf, _ = _per_dispatch(fetch_rows, "copyBuffer", "FETCH_SIZE")
-    cw, _ = _per_dispatch(write_rows, "copyBuffer", "WRITE_SIZE")
+    cf, _ = _per_dispatch(fetch_rows, "copyBuffer", "FETCH_SIZE", largest_grid=True)
+    cw, _ = _per_dispatch(write_rows, "copyBuffer", "WRITE_SIZE", largest_grid=True)
     ff, wf = field / (cf * 1024), field / (cw * 1024)
     traffic = int(round(fetch * 1024 * ff + write * 1024 * wf))
-    names = sorted({k for k, _, c, _ in fetch_rows if PERSISTENT_KERNEL in k})
+    names = sorted({r[0] for r in fetch_rows if PERSISTENT_KERNEL in r[0]})
     try:
         commit = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short=12", "HEAD"], capture_output=True,
                                 text=True).stdout.strip() or None
