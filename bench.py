@@ -112,7 +112,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j3d-shapes", "j2d-storage", "batch2d", "cg",
-                                                  "j3d-tune"],
+                                                  "cg-counters", "j3d-tune"],
                     default="j3d")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--record-profile", nargs="+", metavar="DIR",
@@ -618,6 +618,76 @@ def cg_bytes_per_iter(fmt, n, nnz, vbytes):
     return 7 * n * vbytes + 6 * n * vbytes
 
 
+def cg_skewed(torch, P, args):
+    """Load balance on an irregular matrix: CG (FP64, CSR, 200 iterations) on
+    a symmetric, diagonally dominant matrix with power-law row lengths and
+    three hub rows of 5*10^4 to 1.5*10^5 entries (split across the grid),
+    next to a control matrix with the same rows, about the same entries and
+    the same random columns but every row alike.  Equal times per iteration
+    mean the work split absorbs the skew."""
+    rowptr, col, val = P.skewed_spd_csr(1_000_000, args.seed, hubs=(150_000, 100_000, 50_000), max_deg=200)
+    out = cg_csr_row(torch, P, args, rowptr, col, val)
+    # the control: the same rows and about the same entries, every row alike
+    rowptr, col, val = P.skewed_spd_csr(1_000_000, args.seed, degree=max(1, round(len(col) / 2e6)))
+    ctl = cg_csr_row(torch, P, args, rowptr, col, val)
+    out["uniform_control"] = ctl
+    out["time_vs_uniform_control"] = round(out["ms_per_iteration"] / ctl["ms_per_iteration"], 3)
+    return out
+
+
+def cg_csr_row(torch, P, args, rowptr, col, val):
+    import numpy as np
+    n, nnz = len(rowptr) - 1, len(col)
+    A = dict(rowptr=torch.from_numpy(rowptr).cuda(), col=torch.from_numpy(col).cuda(),
+             val=torch.from_numpy(val).cuda(), nnz=nnz)
+    b = torch.empty(n, dtype=torch.float64, device="cuda")
+    P.fill_uniform(b, n, args.seed, -1.0, 1.0, dtype="f64")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    P.cg_solve("csr", n, b, x, tol=1e-10, max_iter=200, static_iter=True, **A)  # warm-up
+    x.zero_()
+    torch.cuda.synchronize()
+    res = P.cg_solve("csr", n, b, x, tol=1e-10, max_iter=200, static_iter=True, **A)
+    lens = np.diff(rowptr)
+    per_iter = cg_bytes_per_iter("csr", n, nnz, 8)
+    return {"n": n, "nnz": nnz, "max_row": int(lens.max()), "median_row": float(np.median(lens)),
+            "rows_over_32": int((lens > 32).sum()), "rows_over_4096": int((lens > 4096).sum()),
+            "iterations": res.iterations, "ms_per_iteration": round(res.elapsed_ms / max(1, res.iterations), 4),
+            "achieved_gbs": round(per_iter * res.iterations / (res.elapsed_ms / 1e3) / 1e9, 1),
+            "long_rows": res.long_rows, "grid": res.grid}
+
+
+def run_cg_counters(args):
+    """The four FP64 solves whose counters profiles/cg_poisson_256.md reports,
+    one launch each and in this order: CSR cached, CSR uncached, DIA cached,
+    DIA uncached (256^3 Poisson, to ||r||/||b|| < 1e-10)."""
+    import torch
+    import perks_amd as P
+
+    m = 256 if args.size == 512 else args.size
+    n = m ** 3
+    nnz = P.poisson7_nnz(m, m, m)
+    b = torch.empty(n, dtype=torch.float64, device="cuda")
+    P.fill_uniform(b, n, args.seed, -1.0, 1.0, dtype="f64")
+    out = []
+    for fmt in ("csr", "dia"):
+        if fmt == "csr":
+            A = dict(rowptr=torch.empty(n + 1, dtype=torch.int32, device="cuda"),
+                     col=torch.empty(nnz, dtype=torch.int32, device="cuda"),
+                     val=torch.empty(nnz, dtype=torch.float64, device="cuda"), nnz=nnz)
+            P.poisson7_csr(m, m, m, A["rowptr"], A["col"], A["val"])
+        else:
+            val = torch.empty(7 * n, dtype=torch.float64, device="cuda")
+            A = dict(val=val, offsets=P.poisson7_dia(m, m, m, val))
+        for cache in (True, False):
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            res = P.cg_solve(fmt, n, b, x, tol=1e-10, x0_is_zero=True, cache_vector=cache, cache_matrix=cache, **A)
+            out.append({"fmt": fmt, "cache": cache, "iterations": res.iterations, "ms": round(res.elapsed_ms, 3),
+                        "lds_bytes": res.lds_bytes, "cache_units": res.cache_units, "grid": res.grid})
+        del A
+        torch.cuda.empty_cache()
+    print(json.dumps({"workload": "cg-counters", "n": n, "solves": out}), flush=True)
+
+
 def run_cg(args):
     import torch
     import perks_amd as P
@@ -637,14 +707,15 @@ def run_cg(args):
         else:
             val = torch.empty(7 * n, dtype=torch.float64, device="cuda")
             A = dict(val=val, offsets=P.poisson7_dia(m, m, m, val))
-        for prec in ("f64", "f32_refine"):
+        for prec, cache in (("f64", True), ("f64", False), ("f32_refine", True)):
             x = torch.zeros(n, dtype=torch.float64, device="cuda")
-            P.cg_solve(fmt, n, b, x, tol=1e-10, precision=prec, **A)  # warm-up
+            kw = dict(tol=1e-10, precision=prec, cache_vector=cache, cache_matrix=cache)
+            P.cg_solve(fmt, n, b, x, **kw, **A)  # warm-up
             times = []
             for _ in range(max(1, args.steps)):
                 x.zero_()
                 torch.cuda.synchronize()
-                res = P.cg_solve(fmt, n, b, x, tol=1e-10, precision=prec, **A)
+                res = P.cg_solve(fmt, n, b, x, **kw, **A)
                 times.append(res.elapsed_ms)
             ms = sum(times) / len(times)
             # independent FP64 residual with torch on the 3D grid view
@@ -659,15 +730,18 @@ def run_cg(args):
             true_rel = ((b - ax.reshape(-1)).norm() / b.norm()).item()
             vb = 8 if prec == "f64" else 4
             per_iter = cg_bytes_per_iter(fmt, n, nnz, vb)
-            rows[f"{fmt}_{prec}"] = {
+            rows[f"{fmt}_{prec}" + ("" if cache else "_nocache")] = {
                 "code": res.code, "iterations": res.iterations, "outer_iterations": res.outer_iterations,
                 "time_to_tolerance_ms": round(ms, 3), "ms_per_iteration": round(ms / max(1, res.iterations), 4),
                 "final_rel_residual_fp64": res.relative_residual, "true_rel_residual_fp64": true_rel,
                 "grid": res.grid,
                 "achieved_gbs": round(per_iter * res.iterations / (ms / 1e3) / 1e9, 1),
+                "lds_cache_bytes_per_workgroup": res.lds_bytes, "cached_vector_units": res.cache_units,
+                "cached_matrix": res.cache_matrix,
             }
         del A
         torch.cuda.empty_cache()
+    rows["skewed_csr_f64"] = cg_skewed(torch, P, args)
     r = rows["csr_f64"]
     achieved = r["achieved_gbs"]
     traffic = None
@@ -785,7 +859,8 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     {"j3d": run_j3d, "j3d-storage": run_j3d_storage, "j3d-shapes": run_j3d_shapes, "j2d-storage": run_j2d_storage,
-     "batch2d": run_batch2d, "cg": run_cg, "j3d-tune": run_j3d_tune}[args.workload](args)
+     "batch2d": run_batch2d, "cg": run_cg, "cg-counters": run_cg_counters,
+     "j3d-tune": run_j3d_tune}[args.workload](args)
 
 
 if __name__ == "__main__":

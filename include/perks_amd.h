@@ -367,7 +367,11 @@ typedef struct perks_cg_opts {
     int32_t x0_is_zero;  /* 1: skip the initial A x (x must hold zeros)    */
     int32_t blkpsm;      /* cap on workgroups per CU (0 = occupancy)       */
     int32_t static_iter; /* 1: run exactly max_iter iterations (--staticiter, cg_kernels.cuh:834) */
-    int32_t reserved[4];
+    int32_t cache_vector;/* 0: keep the workgroups' own rows of Ap, x, p, r in LDS across
+                            iterations while LDS lasts (cacheVector); -1: off            */
+    int32_t cache_matrix;/* 0: keep the leading matrix entries (CSR) or rows (DIA) of each
+                            workgroup in LDS with what LDS is left (cacheMatrix); -1: off */
+    int32_t reserved[2];
 } perks_cg_opts;
 
 typedef struct perks_cg_report {
@@ -381,7 +385,11 @@ typedef struct perks_cg_report {
     double* history;          /* host array: r.r after each iteration (F64:   */
                               /* [0] initial) or each refinement step        */
     int32_t hist_len;         /* entries produced (may exceed hist_cap)      */
-    int32_t reserved[5];
+    int32_t lds_bytes;        /* LDS cache per workgroup (bytes)             */
+    int32_t cache_units;      /* 256-row vector units cached per workgroup   */
+    int32_t cache_matrix;     /* CSR entries / DIA rows cached per workgroup */
+    int32_t long_rows;        /* CSR rows split across the grid (> 64 entries) */
+    int32_t reserved[1];
 } perks_cg_report;
 
 void perks_cg_opts_init(perks_cg_opts* opts);

@@ -360,13 +360,41 @@ void po_poisson7_dia(int nx, int ny, int nz, int32_t* offsets, double* val)
 }
 
 /* y_i = sum over stored entries of row i, in storage order, starting from 0. */
+/* CSR row sums in the library's fixed order (header: po_csr_spmv_f64).  The
+ * same text serves double and float accumulation. */
+#define PO_CSR_ROW(T, SUF)                                                                          \
+    static T csr_row_##SUF(const int32_t* col, const T* val, int32_t k0, int32_t k1, const T* x)    \
+    {                                                                                               \
+        if (k1 - k0 <= PO_CSR_SHORT_ROW) {                                                          \
+            T s = 0;                                                                                \
+            for (int32_t k = k0; k < k1; ++k) s = s + val[k] * x[col[k]];                           \
+            return s;                                                                               \
+        }                                                                                           \
+        T row = 0;                                                                                  \
+        for (int32_t c0 = k0; c0 < k1; c0 += PO_CSR_ROW_CHUNK) {                                    \
+            const int32_t c1 = k1 - c0 > PO_CSR_ROW_CHUNK ? c0 + PO_CSR_ROW_CHUNK : k1;             \
+            T acc[64];                                                                              \
+            for (int l = 0; l < 64; ++l) {                                                          \
+                acc[l] = 0;                                                                         \
+                for (int32_t k = c0 + l; k < c1; k += 64) acc[l] = acc[l] + val[k] * x[col[k]];     \
+            }                                                                                       \
+            for (int s = 32; s >= 1; s >>= 1)                                                       \
+                for (int l = 0; l < s; ++l) acc[l] = acc[l] + acc[l + s];                           \
+            row = c0 == k0 ? acc[0] : row + acc[0];                                                 \
+        }                                                                                           \
+        return row;                                                                                 \
+    }
+PO_CSR_ROW(double, f64)
+PO_CSR_ROW(float, f32)
+
 void po_csr_spmv_f64(int n, const int32_t* rowptr, const int32_t* col, const double* val, const double* x, double* y)
 {
-    for (int i = 0; i < n; ++i) {
-        double s = 0.0;
-        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * x[col[k]];
-        y[i] = s;
-    }
+    for (int i = 0; i < n; ++i) y[i] = csr_row_f64(col, val, rowptr[i], rowptr[i + 1], x);
+}
+
+void po_csr_spmv_f32(int n, const int32_t* rowptr, const int32_t* col, const float* val, const float* x, float* y)
+{
+    for (int i = 0; i < n; ++i) y[i] = csr_row_f32(col, val, rowptr[i], rowptr[i + 1], x);
 }
 
 /* Diagonals in offset order; entries whose column falls outside [0,n) are skipped. */
@@ -514,7 +542,7 @@ static void spmv_f32(int fmt, int n, const int32_t* rowptr, const int32_t* col, 
     for (int i = 0; i < n; ++i) {
         float s = 0.0f;
         if (fmt == 0) {
-            for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * x[col[k]];
+            s = csr_row_f32(col, val, rowptr[i], rowptr[i + 1], x);
         } else {
             for (int d = 0; d < ndiag; ++d) {
                 int64_t j = (int64_t)i + col[d];

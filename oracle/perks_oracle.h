@@ -122,7 +122,19 @@ void po_poisson7_csr(int nx, int ny, int nz, int32_t* rowptr, int32_t* col, doub
  * val[d*N + i] = A[i][i+off_d] (0 where i+off_d falls outside or crosses a face). */
 void po_poisson7_dia(int nx, int ny, int nz, int32_t* offsets, double* val);
 
+/* CSR SpMV in the library's fixed per-row order.  A row of at most
+ * PO_CSR_SHORT_ROW entries is summed in storage order from 0.  A longer row is
+ * cut, from its first entry, into chunks of PO_CSR_ROW_CHUNK entries; in a
+ * chunk, lane l (0..63) sums entries l, l+64, l+128, ... in order from 0, the
+ * 64 lane sums are folded pairwise (v[l] += v[l+s] for s = 32, 16, ..., 1) and
+ * v[0] is the chunk's value; the row is chunk 0's value plus each following
+ * chunk's value in order.  (The reference sums rows through CUB's merge-path
+ * agent, cub/agent/agent_spmv_orig.cuh:666-1100, whose order follows its tile
+ * partition; this order depends on the row length only.) */
+#define PO_CSR_SHORT_ROW 64
+#define PO_CSR_ROW_CHUNK 4096
 void po_csr_spmv_f64(int n, const int32_t* rowptr, const int32_t* col, const double* val, const double* x, double* y);
+void po_csr_spmv_f32(int n, const int32_t* rowptr, const int32_t* col, const float* val, const float* x, float* y);
 void po_dia_spmv_f64(int n, int ndiag, const int32_t* offsets, const double* val, const double* x, double* y);
 double po_dot_f64(const double* a, const double* b, size_t n);
 double po_dot_f32(const float* a, const float* b, size_t n);

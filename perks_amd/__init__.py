@@ -6,7 +6,7 @@ entry points; it loads the library and fails loudly if it is missing.
 """
 from . import _lib
 from .cg import (CgResult, cg_driver, cg_solve, csr_to_dia, gen_tridiag, mm_read_csr, poisson7_csr, poisson7_dia,
-                 poisson7_nnz)
+                 poisson7_nnz, skewed_spd_csr)
 from ._lib import PerksError, check, load
 
 
@@ -19,7 +19,7 @@ from .stencil import (BatchPlan, fill_uniform, j3d_autotune, j3d_device, j3d_ite
                       jacobi2d_batch, jacobi2d_device, jacobi2d_run, jacobi_iterative, make_opts, plan, query_device, shape_id, shape_info)
 
 __all__ = [
-    "BatchPlan", "CgResult", "cg_driver", "cg_solve", "csr_to_dia", "gen_tridiag", "mm_read_csr", "poisson7_csr", "poisson7_dia", "poisson7_nnz",
+    "BatchPlan", "CgResult", "cg_driver", "cg_solve", "csr_to_dia", "gen_tridiag", "mm_read_csr", "poisson7_csr", "poisson7_dia", "poisson7_nnz", "skewed_spd_csr",
     "PerksError", "check", "load", "release", "fill_uniform", "j3d_autotune", "j3d_device", "j3d_kernel_table", "j3d_iterative", "j3d_run", "jacobi2d_batch",
     "jacobi2d_device", "jacobi2d_run", "jacobi_iterative", "make_opts", "plan", "query_device", "shape_id",
     "shape_info", "_lib",

@@ -111,7 +111,7 @@ class CgOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("relative", C.c_int32), ("max_iter", C.c_int32),
                 ("precision", C.c_int32), ("max_inner", C.c_int32), ("max_outer", C.c_int32),
                 ("x0_is_zero", C.c_int32), ("blkpsm", C.c_int32), ("static_iter", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("cache_vector", C.c_int32), ("cache_matrix", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class CgDriverOpts(C.Structure):
@@ -129,7 +129,9 @@ class CgDriverResult(C.Structure):
 class CgReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32), ("final_rr", C.c_double),
                 ("bb", C.c_double), ("elapsed_ms", C.c_double), ("grid", C.c_int32), ("hist_cap", C.c_int32),
-                ("history", C.POINTER(C.c_double)), ("hist_len", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("history", C.POINTER(C.c_double)), ("hist_len", C.c_int32), ("lds_bytes", C.c_int32),
+                ("cache_units", C.c_int32), ("cache_matrix", C.c_int32), ("long_rows", C.c_int32),
+                ("reserved", C.c_int32 * 1)]
 
 
 _P = C.c_void_p

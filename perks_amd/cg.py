@@ -27,6 +27,10 @@ class CgResult:
     elapsed_ms: float
     grid: int
     history: np.ndarray = field(repr=False)
+    lds_bytes: int = 0     # LDS cache per workgroup
+    cache_units: int = 0   # 256-row vector units (Ap, x, p, r) cached per workgroup
+    cache_matrix: int = 0  # CSR entries / DIA rows cached per workgroup
+    long_rows: int = 0     # CSR rows of more than 64 entries, split across the grid
 
     @property
     def relative_residual(self) -> float:
@@ -49,26 +53,31 @@ def poisson7_dia(nx, ny, nz, val, stream=None) -> np.ndarray:
     return offs
 
 
-def _opts(tol, relative, max_iter, precision, inner_tol, max_inner, max_outer, x0_is_zero, blkpsm, static_iter=0):
+def _opts(tol, relative, max_iter, precision, inner_tol, max_inner, max_outer, x0_is_zero, blkpsm, static_iter=0,
+          cache_vector=True, cache_matrix=True):
     o = L.CgOpts()
     L.load().perks_cg_opts_init(C.byref(o))
     o.tol, o.relative, o.max_iter = float(tol), int(relative), int(max_iter)
     o.precision = L.CG_F32_REFINE if precision in ("f32_refine", L.CG_F32_REFINE) else L.CG_F64
     o.inner_tol, o.max_inner, o.max_outer = float(inner_tol), int(max_inner), int(max_outer)
     o.x0_is_zero, o.blkpsm, o.static_iter = int(x0_is_zero), int(blkpsm), int(static_iter)
+    o.cache_vector, o.cache_matrix = (0 if cache_vector else -1), (0 if cache_matrix else -1)
     return o
 
 
 def cg_solve(fmt, n, b, x, *, rowptr=None, col=None, val=None, nnz=None, offsets=None, tol=1e-10, relative=True,
              max_iter=10000, precision="f64", inner_tol=1e-5, max_inner=0, max_outer=0, x0_is_zero=False,
-             blkpsm=0, stream=None, hist_cap=None, static_iter=False) -> CgResult:
+             blkpsm=0, stream=None, hist_cap=None, static_iter=False, cache_vector=True,
+             cache_matrix=True) -> CgResult:
     """fmt 'csr' (rowptr, col, val, nnz) or 'dia' (offsets: host int32 array,
     val: device ndiag*n).  precision 'f64', 'f32_refine', or 'f32' (FP32
     matrix and vectors: val, b, x float32).  Returns a CgResult; raises on
-    argument errors, returns code PERKS_ERR_NOT_CONVERGED (-14) without raising."""
+    argument errors, returns code PERKS_ERR_NOT_CONVERGED (-14) without raising.
+    cache_vector / cache_matrix: the LDS caches across iterations (the
+    reference's cacheVector / cacheMatrix); they change no arithmetic."""
     lib = L.load()
     o = _opts(tol, relative, max_iter, "f64" if precision == "f32" else precision, inner_tol, max_inner, max_outer,
-              x0_is_zero, blkpsm, static_iter)
+              x0_is_zero, blkpsm, static_iter, cache_vector, cache_matrix)
     cap = hist_cap if hist_cap is not None else max_iter + 2
     hist = np.zeros(max(cap, 1))
     rep = L.CgReport()
@@ -89,10 +98,54 @@ def cg_solve(fmt, n, b, x, *, rowptr=None, col=None, val=None, nnz=None, offsets
     if rc < 0 and rc != L.ERR_NOT_CONVERGED:
         raise L.PerksError(rc, "cg_solve")
     return CgResult(rc, rep.iterations, rep.outer_iterations, rep.final_rr, rep.bb, rep.elapsed_ms, rep.grid,
-                    hist[: min(rep.hist_len, hist.size)].copy())
+                    hist[: min(rep.hist_len, hist.size)].copy(), rep.lds_bytes, rep.cache_units, rep.cache_matrix,
+                    rep.long_rows)
 
 
 # ---- the reference's driver and sparse inputs ----------------------------------
+def skewed_spd_csr(n, seed, hubs=(), max_deg=400, dtype=np.float64, degree=None):
+    """A symmetric, diagonally dominant (so SPD) matrix with power-law row
+    lengths: every row links to a Zipf-distributed number of random columns,
+    each hub row to `hubs[i]` columns; the pattern is symmetrised, off-diagonal
+    values are uniform in [-1, 0) and the diagonal is 1 + the row's absolute sum.
+    Returns (rowptr, col, val) with ascending columns.  degree: every row
+    links to that many random columns instead (the same structure without
+    the skew)."""
+    rng = np.random.default_rng(seed)
+    deg = np.full(n, degree) if degree else np.minimum(rng.zipf(1.6, size=n), max_deg)
+    rows = [np.repeat(np.arange(n), deg)]
+    cols = [rng.integers(0, n, size=int(deg.sum()))]
+    for h, d in enumerate(hubs):
+        hub = (h * 7919 + 13) % n
+        rows.append(np.full(d, hub))
+        cols.append(rng.choice(n, size=d, replace=False))
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    keep = r != c
+    r, c = r[keep], c[keep]
+    r, c = np.concatenate([r, c]), np.concatenate([c, r])
+    key = np.unique(r.astype(np.int64) * n + c)
+    r, c = (key // n).astype(np.int64), (key % n).astype(np.int64)
+    # symmetric values: (i, j) and (j, i) take the value drawn for the unordered pair
+    lo, hi = np.minimum(r, c), np.maximum(r, c)
+    pair = lo * n + hi
+    uniq, inv = np.unique(pair, return_inverse=True)
+    pv = -rng.uniform(1e-3, 1.0, size=len(uniq))
+    v = pv[inv]
+    diag = np.zeros(n)
+    np.add.at(diag, r, np.abs(v))
+    r = np.concatenate([r, np.arange(n)])
+    c = np.concatenate([c, np.arange(n)])
+    v = np.concatenate([v, diag + 1.0])
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    rowptr = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(rowptr, r + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    return rowptr, c.astype(np.int32), v.astype(dtype)
+
+
+
 def cg_driver(mtx=None, value_bytes=8, vdata_n=0, max_iter=0, static_iter=False, warmup=False, check=False,
               verbose=False, tol=0.0) -> L.CgDriverResult:
     """myTest<ValueT,int,...> (conjugateGradient/cg_driver.cu:53-428) on this

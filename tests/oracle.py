@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
             "po_poisson7_csr": (None, [I, I, I, P, P, P]),
             "po_poisson7_dia": (None, [I, I, I, P, P]),
             "po_csr_spmv_f64": (None, [I, P, P, P, P, P]),
+            "po_csr_spmv_f32": (None, [I, P, P, P, P, P]),
             "po_dia_spmv_f64": (None, [I, I, P, P, P, P]),
             "po_dot_f64": (D, [P, P, S]),
             "po_dot_f32": (D, [P, P, S]),
@@ -290,3 +291,15 @@ def cg_f32_refine(fmt, n, rowptr, col, val, ndiag, b, tol, relative, inner_tol, 
                                    p(x), tol, int(relative), inner_tol, max_inner, max_outer, p(hist), hist.size,
                                    C.byref(outer))
     return x, hist[: outer.value + 1], total, outer.value
+
+
+def csr_spmv(rowptr, col, val, x):
+    """The oracle's CSR SpMV (fixed per-row order), FP64 or FP32 by val's dtype."""
+    n = len(rowptr) - 1
+    y = np.empty(n, dtype=val.dtype)
+    f = lib().po_csr_spmv_f64 if val.dtype == np.float64 else lib().po_csr_spmv_f32
+    f(n, p(np.ascontiguousarray(rowptr, np.int32)), p(np.ascontiguousarray(col, np.int32)), p(val), p(x), p(y))
+    return y
+
+
+from perks_amd.cg import skewed_spd_csr  # noqa: E402,F401  (test matrices; numpy only)

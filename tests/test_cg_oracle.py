@@ -11,6 +11,7 @@ rounded sum, and CG's finite-termination property on a tiny system.
 import math
 
 import numpy as np
+import pytest
 
 import oracle as O
 
@@ -105,3 +106,107 @@ def test_refinement_reaches_fp64_tolerance():
     assert 2 <= outer <= 8
     want, _, _ = O.cg(0, n, rowptr, col, val, 0, b, 1e-13, True, 2000)
     np.testing.assert_allclose(x, want, rtol=0, atol=1e-8)
+
+
+# ---- the CSR row order (po_csr_spmv_*) ----------------------------------------------
+def _row(values):
+    """One CSR row holding `values` against x = 1 (products = values)."""
+    n = len(values)
+    return (np.array([0, n], np.int32), np.arange(n, dtype=np.int32) % 1, np.asarray(values, np.float64),
+            np.ones(1))
+
+
+BIG = 2.0 ** 53  # next to it the spacing is 2, and BIG + 1 is a tie that rounds back to BIG
+
+
+def _seq(vals):
+    s = 0.0
+    for v in vals:
+        s = s + v
+    return s
+
+
+def test_short_rows_sum_in_storage_order():
+    vals = [BIG] + [1.0] * 20 + [-BIG] + [1.0] * 5
+    rp, col, val, x = _row(vals)
+    assert len(val) <= 64
+    assert O.csr_spmv(rp, col, val, x)[0] == 5.0 == _seq(vals)
+
+
+def test_medium_rows_sum_lane_strided_then_pairwise():
+    """100 entries: lane l sums entries l and l+64.  With BIG at entry 0 and
+    -BIG at entry 64, lane 0 is exactly 0 and the other 98 ones survive: 98,
+    the exact sum.  Storage order gives 35 (the 63 ones after BIG are lost)."""
+    vals = [1.0] * 100
+    vals[0], vals[64] = BIG, -BIG
+    rp, col, val, x = _row(vals)
+    assert _seq(vals) == 35.0
+    assert O.csr_spmv(rp, col, val, x)[0] == 98.0
+
+
+def test_long_rows_fold_chunk_values_in_order():
+    """4096 + 64 entries, BIG at entry 0 (chunk 0, lane 0) and -BIG at entry
+    4097 (chunk 1, lane 1), ones elsewhere.  Chunk 0 is BIG + 4032 (lane 0's
+    63 ones are lost; the other lanes' 4032 ones fold in exactly), chunk 1 is
+    -BIG + 63, exactly (magnitudes below 2^53), and the row is 4095, the exact
+    sum.  Storage order gives 62."""
+    vals = [1.0] * (4096 + 64)
+    vals[0], vals[4097] = BIG, -BIG
+    rp, col, val, x = _row(vals)
+    assert _seq(vals) == 62.0
+    assert O.csr_spmv(rp, col, val, x)[0] == 4095.0
+
+
+def _row_order_numpy(v, short=64, chunk=4096):
+    """The documented order restated with numpy arrays (lane sums as columns of
+    a padded [k, 64] array, summed row by row)."""
+    if len(v) <= short:
+        s = v.dtype.type(0)
+        for t in v:
+            s = s + t
+        return s
+    out = None
+    for c0 in range(0, len(v), chunk):
+        part = v[c0:c0 + chunk]
+        pad = np.zeros(-(-len(part) // 64) * 64, v.dtype)
+        pad[:len(part)] = part
+        m = pad.reshape(-1, 64)
+        acc = np.zeros(64, v.dtype)
+        for j in range(m.shape[0]):
+            live = np.arange(64) + 64 * j < len(part)
+            acc = np.where(live, acc + m[j], acc)
+        s = 32
+        while s >= 1:
+            acc[:s] = acc[:s] + acc[s:2 * s]
+            s //= 2
+        out = acc[0] if out is None else out + acc[0]
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_row_order_against_a_numpy_restatement(dtype):
+    rng = np.random.default_rng(11)
+    lens = [0, 1, 31, 32, 33, 63, 64, 65, 100, 4095, 4096, 4097, 8192, 8193, 10000]
+    rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    n = 12000
+    col = rng.integers(0, n, size=rowptr[-1]).astype(np.int32)
+    val = rng.uniform(-1, 1, size=rowptr[-1]).astype(dtype)
+    x = rng.uniform(-1, 1, size=n).astype(dtype)
+    y = O.csr_spmv(rowptr, col, val, x)
+    for i in range(len(lens)):
+        k0, k1 = rowptr[i], rowptr[i + 1]
+        want = _row_order_numpy(val[k0:k1] * x[col[k0:k1]])
+        assert y[i].tobytes() == np.asarray(want, dtype).tobytes(), (i, lens[i])
+
+
+def test_skewed_matrix_is_spd_and_cg_solves_it():
+    rowptr, col, val = O.skewed_spd_csr(3000, 5, hubs=(2500,))
+    n = len(rowptr) - 1
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, col[rowptr[i]:rowptr[i + 1]]] = val[rowptr[i]:rowptr[i + 1]]
+    assert np.array_equal(A, A.T)
+    assert np.diff(rowptr).max() > 4096 // 2
+    b = O.uniform(n, 3, -1.0, 1.0)
+    x, iters, hist = O.cg(0, n, rowptr, col, val, 0, b, 1e-10, 1, 1000)
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-9
