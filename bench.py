@@ -7,7 +7,10 @@ Workload (one bench "step" = one complete solve, inputs already in HBM):
   uniform[0,1) from a seed.  The persistent method runs all 1000 time steps
   in one cooperative launch; the per-step method (one launch per time step)
   is timed on the same GPU and buffers for comparison, and the two results
-  are compared bit for bit.
+  are compared bit for bit.  config.contracted times the same solve with the
+  contracted update (opts.tune[6] = 1: products fused into the running sum,
+  within 4 ulp of the gold per step, not bit-identical) and reports its
+  deviation from the bit-identical result; `value` is the bit-identical form.
 
 Prints one JSON line (see the keys at the bottom).  GCells/s counts
 time steps * cells / seconds, the reference's own definition
@@ -60,6 +63,7 @@ METRIC = "GCells/s + % of HBM roofline, 3d7pt fp64 512³, persistent vs per-step
 HBM_PEAK_GBS = 8000.0
 CONFIG_COEFS = [0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4]  # E W N S T B, centre
 TRAFFIC_FILE = ROOT / "profiles" / "j3d_persistent_f64_512_traffic.json"
+TRAFFIC_FILE_FUSED = ROOT / "profiles" / "j3d_persistent_fused_f64_512_traffic.json"  # --contracted
 PERSISTENT_KERNEL = "j3d_star1_persistent_kernel"
 # Files that define the persistent 7-point kernel and its launch: a recorded
 # profile applies only while their contents are unchanged.
@@ -109,6 +113,9 @@ def parse():
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-per-step", action="store_true", help="skip the per-step comparison")
+    ap.add_argument("--no-contracted", action="store_true", help="skip the contracted-update (FMA) leg")
+    ap.add_argument("--contracted", action="store_true",
+                    help="time the contracted update (opts.tune[6] = 1) as the main leg (for its counter profile)")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j3d-shapes", "j2d-storage", "batch2d", "cg",
@@ -195,10 +202,11 @@ def copy_bandwidth(torch, like) -> float:
     return 2 * like.numel() * like.element_size() / (ms * 1e-3) / 1e9
 
 
-def load_traffic(n, tsteps, dtype):
-    """The recorded counter profile of the persistent kernel, or (None, why)."""
+def load_traffic(n, tsteps, dtype, contracted=False):
+    """The recorded counter profile of the persistent kernel (of its contracted
+    form), or (None, why)."""
     try:
-        d = json.loads(TRAFFIC_FILE.read_text())
+        d = json.loads((TRAFFIC_FILE_FUSED if contracted else TRAFFIC_FILE).read_text())
     except (OSError, ValueError):
         return None, "no recorded profile"
     if d.get("size") != n or d.get("time_steps") != tsteps or d.get("dtype") != dtype:
@@ -221,6 +229,8 @@ def run_j3d(args):
     init = torch.empty(cells, dtype=tdt, device="cuda")
     P.fill_uniform(init, cells, 1, 0.0, 1.0, dtype=args.dtype)
     opts = dict(boundary="zero", coefs=CONFIG_COEFS)
+    if args.contracted:
+        opts["tune"] = [0, 0, 0, 0, 0, 0, 1]
 
     def solve(method):
         a.copy_(init)
@@ -262,8 +272,44 @@ def run_j3d(args):
                                                             persistent_result.view(torch.int64 if esz == 8
                                                                                    else torch.int32)))
 
+    if args.method == "persistent" and not args.no_contracted and not args.contracted:
+        # The contracted update (opts.tune[6] = 1): the gold's term order with
+        # each product fused into the running sum; not the gold's bits.
+        fused = dict(opts, tune=[0, 0, 0, 0, 0, 0, 1])
+        ms = []
+        for i in range(args.warmup + args.steps):
+            a.copy_(init)
+            torch.cuda.synchronize()
+            frep = P.j3d_device(a, b, n, n, n, T, dtype=args.dtype, method="persistent", **fused)
+            if i >= args.warmup:
+                ms.append(frep.elapsed_ms)
+        fres = (b if frep.result_in_b else a).double()
+        ref = persistent_result.double()
+        dev = ((fres - ref).abs().max() / ref.abs().max()).item()
+        fval = len(ms) * T * cells / (sum(ms) / 1e3) / 1e9
+        fr = pass_roofline(PERSISTENT_KERNEL, cells, T, frep.reserved[0], esz, sum(ms) / len(ms) / 1e3)
+        cfg["contracted"] = {
+            "what": "opts.tune[6] = 1: each product after the first fused into the running sum (FMA), gold term "
+                    "order; within 4 ulp of the gold per step, not bit-identical",
+            "gcells_per_s": round(fval, 2), "ms_per_solve": round(sum(ms) / len(ms), 3),
+            "roofline_frac": fr["frac"], "achieved_gbs": fr["achieved"],
+            "max_rel_deviation_vs_uncontracted": float(f"{dev:.3e}"),
+        }
+        if "per_step_kernel_gcells_per_s" in cfg:
+            cfg["contracted"]["speedup_vs_per_step"] = round(fval / cfg["per_step_kernel_gcells_per_s"], 3)
+        fprof, fwhy = load_traffic(n, T, args.dtype, contracted=True)
+        if fprof:
+            cfg["contracted"]["profile"] = {k: fprof[k] for k in ("kernel", "git_commit", "valu_busy",
+                                                                  "valu_lane_insts_per_cell") if k in fprof}
+            cfg["contracted"]["traffic_gbs"] = round(fprof["hbm_bytes_per_launch"] / (sum(ms) / len(ms) / 1e3) / 1e9,
+                                                     1)
+        else:
+            cfg["contracted"]["profile_note"] = fwhy
+        del fres, ref
+
     avg_launch_s = kern_s / args.steps
-    prof, why = load_traffic(n, T, args.dtype) if args.method == "persistent" else (None, "per-step kernel")
+    prof, why = (load_traffic(n, T, args.dtype, args.contracted) if args.method == "persistent"
+                 else (None, "per-step kernel"))
     roofline = pass_roofline(PERSISTENT_KERNEL if args.method == "persistent" else "j3d_star1_plain_kernel",
                              cells, T, levels if args.method == "persistent" else 1, esz, avg_launch_s,
                              prof["hbm_bytes_per_launch"] if prof else None)
@@ -808,6 +854,7 @@ def record_profile(args, out_path=None):
     (__amd_rocclr_copyBuffer, one read and one write of cells * sizeof(T))."""
     import subprocess
     dirs = args.record_profile
+    contracted = bool(getattr(args, "contracted", False))
     n, T = args.size, args.tsteps
     esz = 8 if args.dtype == "f64" else 4
     field = n ** 3 * esz
@@ -819,6 +866,8 @@ def record_profile(args, out_path=None):
     ff, wf = field / (cf * 1024), field / (cw * 1024)
     traffic = int(round(fetch * 1024 * ff + write * 1024 * wf))
     names = sorted({r[0] for r in fetch_rows if PERSISTENT_KERNEL in r[0]})
+    if len(names) > 1:
+        raise SystemExit(f"several persistent kernels in the counter run ({names}): profile with --no-contracted")
     try:
         commit = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short=12", "HEAD"], capture_output=True,
                                 text=True).stdout.strip() or None
@@ -830,8 +879,10 @@ def record_profile(args, out_path=None):
          "calibration": f"__amd_rocclr_copyBuffer of the {field}-byte field in the same runs: FETCH_SIZE {cf:.1f} KiB, "
                         f"WRITE_SIZE {cw:.1f} KiB",
          "source_sha256": kernel_sources_sha(), "git_commit": commit,
+         "contracted": contracted,
          "command": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE, then SQ_INSTS_VALU SQ_ACTIVE_INST_VALU "
-                    "SQ_BUSY_CYCLES) -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-per-step"}
+                    "SQ_BUSY_CYCLES) -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-per-step "
+                    + ("--contracted" if contracted else "--no-contracted")}
     if len(dirs) > 2:
         sq = _counter_rows(dirs[2])
         insts, _ = _per_dispatch(sq, PERSISTENT_KERNEL, "SQ_INSTS_VALU")
@@ -840,7 +891,7 @@ def record_profile(args, out_path=None):
         # SQ_ACTIVE_INST_VALU counts quad-cycles summed over SIMDs; SQ_BUSY_CYCLES is summed over shader engines.
         d["valu_busy"] = round(4 * active / (SIMDS * busy / SHADER_ENGINES), 4)
         d["valu_lane_insts_per_cell"] = round(insts * 64 / (T * n ** 3), 2)
-    (out_path or TRAFFIC_FILE).write_text(json.dumps(d, indent=1) + "\n")
+    (out_path or (TRAFFIC_FILE_FUSED if contracted else TRAFFIC_FILE)).write_text(json.dumps(d, indent=1) + "\n")
     print(json.dumps(d))
 
 

@@ -113,7 +113,12 @@ typedef struct perks_stencil_opts {
      *           (64x32 FP64), 4 = row streaming, two columns per lane (FP32/FP16/BF16 storage,
      *           even widths; otherwise PERKS_ERR_UNSUPPORTED)
      *   tune[5] 3D: 1 = run the 7-point star on the shape-generic kernel
-     *   tune[6..7] unused */
+     *   tune[6] 3D 7-point persistent, FP64/FP32 storage, no divisor: 1 = contracted
+     *           update (the gold's term order with each product fused into the running
+     *           sum by an FMA; not the gold's bits, within a few ulp of them per step);
+     *           PERKS_ERR_UNSUPPORTED for any other method, shape, format or a divisor.
+     *           0 (default) = uncontracted, bit-identical to the reference CPU gold
+     *   tune[7] unused */
     int32_t tune[8];
     /* FP16/BF16 storage only (no reference counterpart): how each step's FP32
      * result is rounded to the storage format.  PERKS_ROUND_NEAREST (default):

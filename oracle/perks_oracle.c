@@ -169,6 +169,55 @@ double po_rms_error_##SUF(const T* a, const T* b, int ny, int nx, int z0, int z1
 DEFINE_STEP(f64, double)
 DEFINE_STEP(f32, float)
 
+/* Contracted update (this library's tune[6] = 1 option; no reference
+ * counterpart): the same group order as step_planes_*, each group's sum of
+ * points formed as there, the first product rounded on its own and every
+ * later product fused into the running sum, acc = fma(coef, s, acc), one
+ * rounding per term; then the optional division.  Groups that the gold
+ * forms in double (prod_f64, FP32 only) are not supported: returns -1. */
+#define DEFINE_FUSED(SUF, T, FMA)                                                             \
+int po_stencil_iterate_fused_##SUF(const T* in, T* out, int nz, int ny, int nx, const po_stencil* st, int steps) \
+{                                                                                             \
+    for (int g = 0; g < st->ngroups; ++g)                                                     \
+        if (sizeof(T) != sizeof(double) && st->groups[g].prod_f64) return -1;                 \
+    const size_t plane = (size_t)ny * nx, n = (size_t)nz * plane;                             \
+    if (steps <= 0) { memcpy(out, in, n * sizeof(T)); return 0; }                             \
+    T* tmp = (T*)malloc(n * sizeof(T));                                                       \
+    T* dst = (steps % 2 == 1) ? out : tmp;                                                    \
+    T* other = (steps % 2 == 1) ? tmp : out;                                                  \
+    const T* src = in;                                                                        \
+    const int bc = st->boundary;                                                              \
+    for (int it = 0; it < steps; ++it) {                                                      \
+        for (int z = 0; z < nz; ++z)                                                          \
+            for (int y = 0; y < ny; ++y)                                                      \
+                for (int x = 0; x < nx; ++x) {                                                \
+                    T acc = 0;                                                                \
+                    for (int g = 0; g < st->ngroups; ++g) {                                   \
+                        const po_group* gr = &st->groups[g];                                  \
+                        T s = 0;                                                              \
+                        for (int p = 0; p < gr->npts; ++p) {                                  \
+                            int zz = nb(z, gr->off[p][0], nz, bc);                            \
+                            int yy = nb(y, gr->off[p][1], ny, bc);                            \
+                            int xx = nb(x, gr->off[p][2], nx, bc);                            \
+                            T v = (zz < 0 || yy < 0 || xx < 0) ? (T)0                         \
+                                                               : src[zz * plane + (size_t)yy * nx + xx]; \
+                            s = (p == 0) ? v : (T)(s + v);                                    \
+                        }                                                                     \
+                        acc = (g == 0) ? (T)((T)gr->coef * s) : FMA((T)gr->coef, s, acc);     \
+                    }                                                                         \
+                    if (st->divisor != 0.0) acc = (T)(acc / (T)st->divisor);                  \
+                    dst[z * plane + (size_t)y * nx + x] = acc;                                \
+                }                                                                             \
+        src = dst;                                                                            \
+        T* t = dst; dst = other; other = t;                                                   \
+    }                                                                                         \
+    free(tmp);                                                                                \
+    return 0;                                                                                 \
+}
+
+DEFINE_FUSED(f64, double, fma)
+DEFINE_FUSED(f32, float, fmaf)
+
 /* ------------------------------------------------------------------------ */
 /* Reduced-precision storage                                                 */
 /* ------------------------------------------------------------------------ */

@@ -74,6 +74,11 @@ void po_stencil_iterate_f32(const float* in, float* out, int nz, int ny, int nx,
 /* Same, with the z loop split across `threads` OpenMP threads (bitwise equal). */
 void po_stencil_iterate_omp_f64(const double* in, double* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
 void po_stencil_iterate_omp_f32(const float* in, float* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
+/* Contracted form (no reference counterpart): group sums as above, the first
+ * product rounded alone, each later one fused, acc = fma(coef, s, acc).
+ * Returns -1 (nothing written) for FP32 stencils with double-formed groups. */
+int po_stencil_iterate_fused_f64(const double* in, double* out, int nz, int ny, int nx, const po_stencil* st, int steps);
+int po_stencil_iterate_fused_f32(const float* in, float* out, int nz, int ny, int nx, const po_stencil* st, int steps);
 
 /* Reduced-precision storage: fields held as IEEE binary16 ("f16s") or bfloat16
  * ("bf16s") bit patterns, each step computed in float exactly as the f32 path

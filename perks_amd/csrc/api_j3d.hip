@@ -111,6 +111,11 @@ int validate(const T* a, const T* b, int nz, int ny, int nx, int steps, const pe
     if (o->rounding != PERKS_ROUND_NEAREST && o->rounding != PERKS_ROUND_STOCHASTIC) return PERKS_ERR_INVALID_ARG;
     const ShapeDesc* s = shape_by_id(o->shape);
     if (!s || s->dims != 3) return PERKS_ERR_INVALID_ARG;
+    // tune[6] = 1: the contracted 7-point update, persistent kernel, FP64/FP32 storage
+    if (o->tune[6] != 0) {
+        if (o->tune[6] != 1) return PERKS_ERR_INVALID_ARG;
+        if (sizeof(T) == 2 || uses_general(o) || o->method != PERKS_METHOD_PERSISTENT) return PERKS_ERR_UNSUPPORTED;
+    }
     return PERKS_OK;
 }
 

@@ -48,6 +48,26 @@ __device__ __forceinline__ T star1_update(const T (&k)[8], T e, T w, T n, T s, T
     return acc;
 }
 
+// The same sum with each product fused into the running sum (one rounding
+// per term instead of two): acc = k0 e, then acc = fma(k_i, x_i, acc) in the
+// gold's term order.  Not the gold's bits; within a few ulp of them per step.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename T, bool HAS_DIV>
+__device__ __forceinline__ T star1_update_fused(const T (&k)[8], T e, T w, T n, T s, T t, T b, T c)
+{
+    T acc = k[0] * e;
+    acc = fma_t(k[1], w, acc);
+    acc = fma_t(k[2], n, acc);
+    acc = fma_t(k[3], s, acc);
+    acc = fma_t(k[4], t, acc);
+    acc = fma_t(k[5], b, acc);
+    acc = fma_t(k[6], c, acc);
+    if (HAS_DIV) acc = acc / k[7];
+    return acc;
+}
+
 template <typename T>
 __device__ __forceinline__ void load_coefs7(const KCoefs& cf, T (&k)[8])
 {

@@ -19,8 +19,9 @@ int release_persistent_scratch();
 
 // Number of tile variants (opts.tune[4]) of the persistent 7-point kernel.
 int j3d_persistent_variants();
-// The persistent kernel of tile variant `var`, and its workgroup size in
-// *block, for attribute and occupancy queries; nullptr when not built.
-void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, int* block);
+// The persistent kernel of tile variant `var` (fused: the contracted update),
+// and its workgroup size in *block, for attribute and occupancy queries;
+// nullptr when not built.
+void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, bool fused, int* block);
 
 }  // namespace perks

@@ -54,13 +54,13 @@ using j3dp::PersistArgs;
 using j3dp::Variant;
 using j3dp::variant_shape;
 
-void* pick_kernel(int fmt, int bc, bool has_div, int var, bool sr)
+void* pick_kernel(int fmt, int bc, bool has_div, int var, bool sr, bool fused)
 {
     switch (fmt) {
-    case PERKS_F64: return j3d_persistent_kernel_f64(bc, has_div, var, sr);
-    case PERKS_F32: return j3d_persistent_kernel_f32(bc, has_div, var, sr);
-    case PERKS_F16: return j3d_persistent_kernel_f16(bc, has_div, var, sr);
-    case PERKS_BF16: return j3d_persistent_kernel_bf16(bc, has_div, var, sr);
+    case PERKS_F64: return j3d_persistent_kernel_f64(bc, has_div, var, sr, fused);
+    case PERKS_F32: return j3d_persistent_kernel_f32(bc, has_div, var, sr, fused);
+    case PERKS_F16: return j3d_persistent_kernel_f16(bc, has_div, var, sr, fused);
+    case PERKS_BF16: return j3d_persistent_kernel_bf16(bc, has_div, var, sr, fused);
     default: return nullptr;
     }
 }
@@ -105,8 +105,7 @@ int tile_width(const Variant& v, int l) { return v.pair ? 128 - 4 * ((l + 1) / 2
 // rounds * (planes per item).
 static int best_nzs(int nz, int ny, int ntx, int l, int G, int rows)
 {
-    const int VH = rows - 2 * (l - 1);
-    const long long tiles = (long long)ntx * ((ny + VH - 1) / VH);
+    const long long tiles = (long long)ntx * j3dp::y_tiles(ny, rows, l);
     int best = 1;
     double best_cost = 1e300;
     const int cap = nz < 256 ? nz : 256;
@@ -195,11 +194,11 @@ int min_width_y_persistent(int dtype, int nx, int ny, int, int)
 
 int j3d_persistent_variants() { return kNumVariants; }
 
-void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, int* block)
+void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, bool fused, int* block)
 {
     if (var < 0 || var >= kNumVariants) return nullptr;
     if (block) *block = variant_shape(dtype, var).nw * 64;
-    return pick_kernel(dtype, bc, has_div, var, sr);
+    return pick_kernel(dtype, bc, has_div, var, sr, fused);
 }
 
 // ---- launch -----------------------------------------------------------------
@@ -216,7 +215,9 @@ int run_persistent_fmt(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, 
     if (!caps.cooperative) return PERKS_ERR_NOT_LAUNCHABLE;
     const bool has_div = cf.divisor != 0.0;
     const int var = variant_of(o);
-    void* fn = pick_kernel(dtype, o->boundary, has_div, var, o->rounding == PERKS_ROUND_STOCHASTIC);
+    // tune[6] = 1: the contracted update (FP64/FP32 without a divisor; the
+    // kernel pickers return nullptr for the rest)
+    void* fn = pick_kernel(dtype, o->boundary, has_div, var, o->rounding == PERKS_ROUND_STOCHASTIC, o->tune[6] == 1);
     if (!fn) return PERKS_ERR_UNSUPPORTED;
     int occ = 0;
     PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn),

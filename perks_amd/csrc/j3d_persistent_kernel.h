@@ -128,6 +128,66 @@ __host__ __device__ inline void x_tile(int bc, int nx, int L, int tx, int ntx, i
     }
 }
 
+// Tiling of the y axis for L levels: a workgroup's `rows` rows store
+// VH = rows - 2(L-1), the outermost L-1 rows on each side being ghosts at
+// level L.  Rows on a y face apply the boundary rule by selects at every
+// level, whatever lies beyond, so a tile whose first (last) row is the face
+// row loses no row on that side: the two edge tiles start at the faces and
+// store rows - (L-1) rows each.  At 512 rows, 40 rows per tile and L = 4 that
+// is 37 + 13 x 34 + 37, fifteen tiles instead of sixteen.
+__host__ __device__ inline bool y_face_tiles(int ny, int rows, int L)
+{
+    return ny >= 2 * (rows - (L - 1));
+}
+__host__ __device__ inline int y_tiles(int ny, int rows, int L)
+{
+    const int VH = rows - 2 * (L - 1);
+    if (!y_face_tiles(ny, rows, L)) return (ny + VH - 1) / VH;
+    const int EH = rows - (L - 1);
+    return 2 + (ny - 2 * EH + VH - 1) / VH;
+}
+// Tile ty of nty: stored rows [vy0, vy1) and the row of the workgroup's first row.
+__host__ __device__ inline void y_tile(int ny, int rows, int L, int ty, int nty, int& vy0, int& vy1, int& yw0)
+{
+    const int VH = rows - 2 * (L - 1);
+    if (!y_face_tiles(ny, rows, L)) {
+        vy0 = ty * VH;
+        vy1 = vy0 + VH < ny ? vy0 + VH : ny;
+        yw0 = vy0 - (L - 1);
+        return;
+    }
+    const int EH = rows - (L - 1);
+    if (ty == 0) {
+        vy0 = 0;
+        vy1 = EH;
+        yw0 = 0;
+    } else if (ty == nty - 1) {
+        vy0 = ny - EH;
+        vy1 = ny;
+        yw0 = ny - rows;
+    } else {
+        vy0 = EH + (ty - 1) * VH;
+        vy1 = vy0 + VH < ny - EH ? vy0 + VH : ny - EH;
+        yw0 = vy0 - (L - 1);
+    }
+}
+
+// Work items of one pass: item = t + ntiles * (z-slab) marches planes
+// [zs, ze) of x-y tile t, one of nzs equal z-slabs.  Items are dealt to the
+// workgroups in turn, so at any moment the grid marches neighbouring tiles
+// over neighbouring planes and the ghost rims one tile reads from HBM are
+// still in L2 / the MALL when its neighbours read them.  (Cutting the
+// tile-major (tile, plane) sequence into equal ranges per workgroup instead
+// balances the work exactly but scatters the workgroups over z: 33% slower
+// at 512^3 FP64.)
+__device__ __forceinline__ void item_piece(int item, int ntiles, int nzs, int nz, int& t, int& zs, int& ze)
+{
+    t = item % ntiles;
+    const int zsl = item / ntiles;
+    zs = (int)(((long long)zsl * nz) / nzs);
+    ze = (int)(((long long)(zsl + 1) * nz) / nzs);
+}
+
 // One work item: L time steps over an x-y tile and a z-slab.
 //
 // Lanes 0 and 63 are ghost lanes at level 0 already (no halo column loads):
@@ -165,7 +225,7 @@ __device__ __forceinline__ double read_lane_t<double>(double v, int l)
 // level 0 through the load, higher levels through side_faces), so waves at
 // an x face take the update without boundary tests.  Waves at a y face and
 // the planes at z faces apply the rule by selects (compute<true>).
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
 struct March {
     using E = Elem<FMT>;
     using S = typename E::S;
@@ -235,7 +295,8 @@ struct March {
                 t = zhi ? bv : t;
                 bt = zlo ? bv : bt;
             }
-            const T v = star1_update<T, HAS_DIV>(k, e, w, n, s, t, bt, cc);
+            const T v = FU ? star1_update_fused<T, HAS_DIV>(k, e, w, n, s, t, bt, cc)
+                           : star1_update<T, HAS_DIV>(k, e, w, n, s, t, bt, cc);
             out[r] = E::template qr<SR>(v, (uint32_t)p * plane32 + cbase + (uint32_t)(r * nx), key[lv]);
         }
     }
@@ -362,25 +423,18 @@ struct March {
         return true;
     }
 
-    __device__ __forceinline__ void run(int item, int nzs)
+    // Planes [zs, ze) of tile (tx, ty) of ntx x nty.
+    __device__ __forceinline__ void run(int tx, int ty, int ntx, int nty, int zs, int ze)
     {
-        // The exchange rows of the previous item may still be being read.
+        // The exchange rows of the previous piece may still be being read.
         lds_barrier();
-        const int VH = NW * NY - 2 * (L - 1);
-        const int ntx = x_tiles(BC, nx, L);
-        const int nty = (ny + VH - 1) / VH;
-        const int tx = item % ntx;
-        const int ty = (item / ntx) % nty;
-        const int zsl = item / (ntx * nty);
-        const int zs = (int)(((long long)zsl * nz) / nzs);
-        const int ze = (int)(((long long)(zsl + 1) * nz) / nzs);
-
         int vx0, vx1, xw0;  // stored columns, column of lane 0
         x_tile(BC, nx, L, tx, ntx, vx0, vx1, xw0);
-        const int vy0 = ty * VH, vy1 = min(vy0 + VH, ny);
+        int vy0, vy1, yw0;  // stored rows, row of wave 0's first row
+        y_tile(ny, NW * NY, L, ty, nty, vy0, vy1, yw0);
         const int x = xw0 + lane;
         const int xc = x < 0 ? 0 : (x >= nx ? nx - 1 : x);
-        const int y0 = vy0 - (L - 1) + wave * NY;
+        const int y0 = yw0 + wave * NY;
         xlo = x == 0;
         xhi = x == nx - 1;
         xcol = x;
@@ -478,7 +532,7 @@ struct Elem2<PERKS_BF16> : Elem2x16<PERKS_BF16> {};
 // columns (stores stay on whole pairs; the grid width must be even).  Level
 // structure, the one-iteration-late row exchange and the face handling are
 // those of March.
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
 struct March2 {
     using E = Elem2<FMT>;
     using S = typename Elem<FMT>::S;
@@ -559,8 +613,10 @@ struct March2 {
                 ba = zlo ? bva : ba;
                 bb = zlo ? bvb : bb;
             }
-            T va = star1_update<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca);
-            T vb = star1_update<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb);
+            T va = FU ? star1_update_fused<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca)
+                      : star1_update<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca);
+            T vb = FU ? star1_update_fused<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb)
+                      : star1_update<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb);
             E::template qr<SR>(va, vb, (uint32_t)p * plane32 + cbase + (uint32_t)(r * nx), key[lv]);
             out[r][0] = va;
             out[r][1] = vb;
@@ -687,25 +743,18 @@ struct March2 {
         return true;
     }
 
-    __device__ __forceinline__ void run(int item, int nzs)
+    // Planes [zs, ze) of tile (tx, ty) of nty rows of tiles.
+    __device__ __forceinline__ void run(int tx, int ty, int nty, int zs, int ze)
     {
         lds_barrier();
         constexpr int VW = 128 - 2 * G;
-        const int VH = NW * NY - 2 * (L - 1);
-        const int ntx = (nx + VW - 1) / VW;
-        const int nty = (ny + VH - 1) / VH;
-        const int tx = item % ntx;
-        const int ty = (item / ntx) % nty;
-        const int zsl = item / (ntx * nty);
-        const int zs = (int)(((long long)zsl * nz) / nzs);
-        const int ze = (int)(((long long)(zsl + 1) * nz) / nzs);
-
         const int vx0 = tx * VW, vx1 = min(vx0 + VW, nx);
-        const int vy0 = ty * VH, vy1 = min(vy0 + VH, ny);
+        int vy0, vy1, yw0;
+        y_tile(ny, NW * NY, L, ty, nty, vy0, vy1, yw0);
         const int xw0 = vx0 - G;  // column of lane 0's half a (even)
         x0 = xw0 + 2 * lane;
         const int xc = x0 < 0 ? 0 : (x0 > nx - 2 ? nx - 2 : x0);
-        const int y0 = vy0 - (L - 1) + wave * NY;
+        const int y0 = yw0 + wave * NY;
         xlo = x0 == 0;
         xhi = x0 + 1 == nx - 1;
         pout = x0 < 0 || x0 > nx - 2;
@@ -743,17 +792,19 @@ struct March2 {
     }
 };
 
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
 __device__ __forceinline__ void run_pass2(const typename Elem<FMT>::S* src, typename Elem<FMT>::S* dst, float2* xch,
                                           const PersistArgs& a, const float* k, int lane, int wave, int step0)
 {
     constexpr int VW = 128 - 2 * (2 * ((L + 1) / 2));
-    const int VH = NW * NY - 2 * (L - 1);
-    const int nitems = ((a.nx + VW - 1) / VW) * ((a.ny + VH - 1) / VH) * a.nzs[L];
-    const int Gd = gridDim.x;
-    const int me = xcd_tile_id(blockIdx.x, Gd);
-    for (int item = me; item < nitems; item += Gd) {
-        March2<FMT, BC, NY, NW, P, L, HAS_DIV, SR> m;
+    const int ntx = (a.nx + VW - 1) / VW;
+    const int nty = y_tiles(a.ny, NW * NY, L);
+    const int nitems = ntx * nty * a.nzs[L];
+    const int G = gridDim.x;
+    for (int item = xcd_tile_id(blockIdx.x, G); item < nitems; item += G) {
+        int t, zs, ze;
+        item_piece(item, ntx * nty, a.nzs[L], a.nz, t, zs, ze);
+        March2<FMT, BC, NY, NW, P, L, HAS_DIV, SR, FU> m;
         m.src = src;
         m.dst = dst;
         m.xch = xch;
@@ -771,11 +822,11 @@ __device__ __forceinline__ void run_pass2(const typename Elem<FMT>::S* src, type
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) m.k[i] = k[i];
-        m.run(item, a.nzs[L]);
+        m.run(t % ntx, t / ntx, nty, zs, ze);
     }
 }
 
-template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR>
+template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR, bool FU>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE)))
 j3d_star1_persistent2_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S* buf_b, PersistArgs a, KCoefs cf)
 {
@@ -792,10 +843,10 @@ j3d_star1_persistent2_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S
     for (int pass = 0; pass < npass; ++pass) {
         const int lv = (pass == npass - 1) ? a.steps - pass * L : L;
         switch (lv) {
-        case 1: run_pass2<FMT, BC, NY, NW, P, 1, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 2: run_pass2<FMT, BC, NY, NW, P, 2, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 3: run_pass2<FMT, BC, NY, NW, P, 3, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        default: run_pass2<FMT, BC, NY, NW, P, 4, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 1: run_pass2<FMT, BC, NY, NW, P, 1, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 2: run_pass2<FMT, BC, NY, NW, P, 2, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 3: run_pass2<FMT, BC, NY, NW, P, 3, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        default: run_pass2<FMT, BC, NY, NW, P, 4, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
         }
         S* t = src;
         src = dst;
@@ -804,17 +855,19 @@ j3d_star1_persistent2_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S
     }
 }
 
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
 __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typename Elem<FMT>::S* dst,
                                          typename Elem<FMT>::T* xch, const PersistArgs& a,
                                          const typename Elem<FMT>::T* k, int lane, int wave, int step0)
 {
-    const int VH = NW * NY - 2 * (L - 1);
-    const int nitems = x_tiles(BC, a.nx, L) * ((a.ny + VH - 1) / VH) * a.nzs[L];
+    const int ntx = x_tiles(BC, a.nx, L);
+    const int nty = y_tiles(a.ny, NW * NY, L);
+    const int nitems = ntx * nty * a.nzs[L];
     const int G = gridDim.x;
-    const int me = xcd_tile_id(blockIdx.x, G);
-    for (int item = me; item < nitems; item += G) {
-        March<FMT, BC, NY, NW, P, L, HAS_DIV, SR> m;
+    for (int item = xcd_tile_id(blockIdx.x, G); item < nitems; item += G) {
+        int t, zs, ze;
+        item_piece(item, ntx * nty, a.nzs[L], a.nz, t, zs, ze);
+        March<FMT, BC, NY, NW, P, L, HAS_DIV, SR, FU> m;
         m.src = src;
         m.dst = dst;
         m.xch = xch;
@@ -832,11 +885,11 @@ __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typen
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) m.k[i] = k[i];
-        m.run(item, a.nzs[L]);
+        m.run(t % ntx, t / ntx, ntx, nty, zs, ze);
     }
 }
 
-template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR>
+template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR, bool FU>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE)))
 j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S* buf_b, PersistArgs a, KCoefs cf)
 {
@@ -854,10 +907,10 @@ j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S*
     for (int pass = 0; pass < npass; ++pass) {
         const int lv = (pass == npass - 1) ? a.steps - pass * L : L;
         switch (lv) {
-        case 1: run_pass<FMT, BC, NY, NW, P, 1, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 2: run_pass<FMT, BC, NY, NW, P, 2, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 3: run_pass<FMT, BC, NY, NW, P, 3, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        default: run_pass<FMT, BC, NY, NW, P, 4, HAS_DIV, SR>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 1: run_pass<FMT, BC, NY, NW, P, 1, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 2: run_pass<FMT, BC, NY, NW, P, 2, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 3: run_pass<FMT, BC, NY, NW, P, 3, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        default: run_pass<FMT, BC, NY, NW, P, 4, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
         }
         S* t = src;
         src = dst;
@@ -894,25 +947,25 @@ __host__ __device__ constexpr Variant variant_shape(int fmt, int v)
     return fmt == PERKS_F64 ? kVarF64[v] : kVarF32[v];
 }
 
-template <int FMT, int BC, bool HAS_DIV, int V, bool SR>
+template <int FMT, int BC, bool HAS_DIV, int V, bool SR, bool FU>
 void* kernel_ptr()
 {
     constexpr Variant v = variant_shape(FMT, V);
     if constexpr (v.pair != 0 && FMT != PERKS_F64)
-        return reinterpret_cast<void*>(&j3d_star1_persistent2_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR>);
+        return reinterpret_cast<void*>(&j3d_star1_persistent2_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR, FU>);
     else
-        return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR>);
+        return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR, FU>);
 }
 
-template <int FMT, int BC, bool HAS_DIV, bool SR>
+template <int FMT, int BC, bool HAS_DIV, bool SR, bool FU>
 void* pick_variant(int var)
 {
     switch (var) {
-    case 1: return kernel_ptr<FMT, BC, HAS_DIV, 1, SR>();
-    case 2: return kernel_ptr<FMT, BC, HAS_DIV, 2, SR>();
-    case 3: return kernel_ptr<FMT, BC, HAS_DIV, 3, SR>();
-    case 4: return kernel_ptr<FMT, BC, HAS_DIV, 4, SR>();
-    default: return kernel_ptr<FMT, BC, HAS_DIV, 0, SR>();
+    case 1: return kernel_ptr<FMT, BC, HAS_DIV, 1, SR, FU>();
+    case 2: return kernel_ptr<FMT, BC, HAS_DIV, 2, SR, FU>();
+    case 3: return kernel_ptr<FMT, BC, HAS_DIV, 3, SR, FU>();
+    case 4: return kernel_ptr<FMT, BC, HAS_DIV, 4, SR, FU>();
+    default: return kernel_ptr<FMT, BC, HAS_DIV, 0, SR, FU>();
     }
 }
 
@@ -920,15 +973,26 @@ template <int FMT, bool SR>
 void* pick_bc(int bc, bool has_div, int var)
 {
     if (bc == PERKS_BC_ZERO)
-        return has_div ? pick_variant<FMT, PERKS_BC_ZERO, true, SR>(var) : pick_variant<FMT, PERKS_BC_ZERO, false, SR>(var);
-    return has_div ? pick_variant<FMT, PERKS_BC_CLAMP, true, SR>(var) : pick_variant<FMT, PERKS_BC_CLAMP, false, SR>(var);
+        return has_div ? pick_variant<FMT, PERKS_BC_ZERO, true, SR, false>(var)
+                       : pick_variant<FMT, PERKS_BC_ZERO, false, SR, false>(var);
+    return has_div ? pick_variant<FMT, PERKS_BC_CLAMP, true, SR, false>(var)
+                   : pick_variant<FMT, PERKS_BC_CLAMP, false, SR, false>(var);
 }
 
 // sr: stochastic rounding (16-bit storage only; FP64/FP32 ignore it).
+// fused: the contracted update (FP64/FP32 storage without a divisor only;
+// nullptr otherwise).
 template <int FMT>
-void* pick_kernel(int bc, bool has_div, int var, bool sr)
+void* pick_kernel(int bc, bool has_div, int var, bool sr, bool fused)
 {
-    if constexpr (FMT == PERKS_F16 || FMT == PERKS_BF16) {
+    if constexpr (FMT == PERKS_F64 || FMT == PERKS_F32) {
+        if (fused) {
+            if (has_div) return nullptr;
+            return bc == PERKS_BC_ZERO ? pick_variant<FMT, PERKS_BC_ZERO, false, false, true>(var)
+                                       : pick_variant<FMT, PERKS_BC_CLAMP, false, false, true>(var);
+        }
+    } else {
+        if (fused) return nullptr;
         if (sr) return pick_bc<FMT, true>(bc, has_div, var);
     }
     return pick_bc<FMT, false>(bc, has_div, var);
@@ -937,9 +1001,9 @@ void* pick_kernel(int bc, bool has_div, int var, bool sr)
 }  // namespace j3dp
 
 // Kernel entry points per storage format (j3d_persistent_<fmt>.hip).
-void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant, bool sr);
-void* j3d_persistent_kernel_f32(int bc, bool has_div, int variant, bool sr);
-void* j3d_persistent_kernel_f16(int bc, bool has_div, int variant, bool sr);
-void* j3d_persistent_kernel_bf16(int bc, bool has_div, int variant, bool sr);
+void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant, bool sr, bool fused);
+void* j3d_persistent_kernel_f32(int bc, bool has_div, int variant, bool sr, bool fused);
+void* j3d_persistent_kernel_f16(int bc, bool has_div, int variant, bool sr, bool fused);
+void* j3d_persistent_kernel_bf16(int bc, bool has_div, int variant, bool sr, bool fused);
 
 }  // namespace perks

@@ -86,12 +86,13 @@ int describe(int dtype, int nz, int ny, int nx, const perks_stencil_opts* base, 
     size_t lds = 0;
     if (seven_point(&o)) {
         if (c.method == PERKS_METHOD_PLAIN) {
+            if (o.tune[6] == 1) return out->status = PERKS_ERR_UNSUPPORTED;  // contracted: persistent only
             fn = elem_size(dtype) == 2 ? j3d_star1_plain16_fn(dtype, o.boundary, has_div, sr)
                                        : j3d_star1_plain_fn(dtype, o.boundary, has_div, o.tune[1] != 2);
             out->tile_x = elem_size(dtype) == 2 ? 128 : 64;
             out->tile_y = 32;
         } else {
-            fn = j3d_persistent_fn(dtype, o.boundary, has_div, c.variant, sr, &block);
+            fn = j3d_persistent_fn(dtype, o.boundary, has_div, c.variant, sr, o.tune[6] == 1, &block);
         }
     } else {
         fn = j3g_fn(dtype, &o, &block, &lds);

@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
             "po_stencil_iterate_f32": (None, [P, P, I, I, I, P, I]),
             "po_stencil_iterate_omp_f64": (None, [P, P, I, I, I, P, I, I]),
             "po_stencil_iterate_omp_f32": (None, [P, P, I, I, I, P, I, I]),
+            "po_stencil_iterate_fused_f64": (I, [P, P, I, I, I, P, I]),
+            "po_stencil_iterate_fused_f32": (I, [P, P, I, I, I, P, I]),
             "po_f32_to_f16": (C.c_uint16, [C.c_float]),
             "po_f16_to_f32": (C.c_float, [C.c_uint16]),
             "po_f32_to_bf16": (C.c_uint16, [C.c_float]),
@@ -141,6 +143,18 @@ def iterate(field: np.ndarray, st: PoStencil, steps: int, threads: int = 1) -> n
     else:
         fn = lib().po_stencil_iterate_f64 if f64 else lib().po_stencil_iterate_f32
         fn(p(field), p(out), nz, ny, nx, C.byref(st), steps)
+    return out
+
+
+def iterate_fused(field: np.ndarray, st: PoStencil, steps: int) -> np.ndarray:
+    """The contracted form: each group's product after the first fused into the
+    running sum (po_stencil_iterate_fused_*)."""
+    field = np.ascontiguousarray(field)
+    nz, ny, nx = (1,) + field.shape if field.ndim == 2 else field.shape
+    out = np.empty_like(field)
+    fn = lib().po_stencil_iterate_fused_f64 if field.dtype == np.float64 else lib().po_stencil_iterate_fused_f32
+    if fn(p(field), p(out), nz, ny, nx, C.byref(st), steps) != 0:
+        raise ValueError("contracted form: FP32 stencil with double-formed groups")
     return out
 
 

@@ -65,3 +65,21 @@ def test_committed_profile_matches_the_kernel_sources():
     d = json.loads(bench.TRAFFIC_FILE.read_text())
     assert d["source_sha256"] == bench.kernel_sources_sha(), \
         "profiles/j3d_persistent_f64_512_traffic.json was recorded for other kernel sources: re-record it"
+
+
+def test_committed_contracted_profile_matches_the_kernel_sources():
+    d = json.loads(bench.TRAFFIC_FILE_FUSED.read_text())
+    assert d["contracted"] is True
+    assert d["source_sha256"] == bench.kernel_sources_sha(), \
+        "profiles/j3d_persistent_fused_f64_512_traffic.json was recorded for other kernel sources: re-record it"
+
+
+def test_record_profile_refuses_mixed_kernels(tmp_path):
+    k0 = "void perks::j3dp::j3d_star1_persistent_kernel<0, 1, false>(...)"
+    k1 = "void perks::j3dp::j3d_star1_persistent_kernel<0, 1, true>(...)"
+    for d, c in (("f", "FETCH_SIZE"), ("w", "WRITE_SIZE")):
+        write_csv(tmp_path / d, c, [(1, 4096, "__amd_rocclr_copyBuffer", 8.0), (2, 131072, k0, 1.0),
+                                    (3, 131072, k1, 1.0)])
+    args = types.SimpleNamespace(record_profile=[str(tmp_path / d) for d in "fw"], size=8, tsteps=10, dtype="f64")
+    with pytest.raises(SystemExit):
+        bench.record_profile(args, out_path=tmp_path / "p.json")
