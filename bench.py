@@ -42,6 +42,11 @@ Other workloads (--workload; each prints its own JSON line):
                against solving them one after another with the per-step kernel.
   cg           CG on the 7-point Poisson matrix, 256^3, CSR and DIA, FP64 and
                FP32-inner/FP64-refinement, to ||r||/||b|| < 1e-10.
+  j3d-small    the reference's --small / --checkmindomain search (j3d.driver.cpp:62-156):
+               getminHeight for 512 x 512 and 256 x 256 planes, FP64 and FP32,
+               then the 7-point solve at that height, 1000 steps, with the
+               resident-tile kernel, the temporal-blocking kernel and the
+               per-step kernel, compared bit for bit.
   j3d-tune     the configuration table of the 3D path at 512^3 (VGPRs, spills,
                LDS, occupancy of every method / tile variant / pass depth) and
                the autotuner's timing of each, for the 7-point star in every
@@ -119,7 +124,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j3d-shapes", "j2d-storage", "batch2d", "cg",
-                                                  "cg-counters", "j3d-tune"],
+                                                  "cg-counters", "j3d-tune", "j3d-small"],
                     default="j3d")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--record-profile", nargs="+", metavar="DIR",
@@ -440,6 +445,66 @@ def run_j3d_shapes(args):
                                   v["ms_per_solve"] / 1e3),
     }
     print(json.dumps(out), flush=True)
+
+
+def run_j3d_small(args):
+    import ctypes as C
+    import torch
+    import perks_amd as P
+
+    lib = P.load()
+    T = args.tsteps
+    rows = []
+    for dtype in ("f64", "f32"):
+        dt = 0 if dtype == "f64" else 1
+        tdt = torch.float64 if dtype == "f64" else torch.float32
+        for n in (512, 256):
+            o = P.make_opts(boundary="zero", coefs=CONFIG_COEFS)
+            h = lib.perks_j3d_min_height(dt, n, n, C.byref(o))  # --checkmindomain prints "h wy wx"
+            row = {"dtype": dtype, "checkmindomain": f"{h} {n} {n}", "cells": h * n * n, "time_steps": T}
+            init = torch.empty(h * n * n, dtype=tdt, device="cuda")
+            P.fill_uniform(init, init.numel(), args.seed, 0.0, 1.0, dtype=dtype)
+            a, b = torch.empty_like(init), torch.empty_like(init)
+            results = {}
+            for name, method, tune in (("resident", "persistent", [0, 0, 0, 0, 5]),
+                                       ("temporal_blocking", "persistent", None), ("per_step", "plain", None)):
+                ms = []
+                for i in range(1 + args.steps):
+                    a.copy_(init)
+                    torch.cuda.synchronize()
+                    rep = P.j3d_device(a, b, h, n, n, T, dtype=dtype, method=method, boundary="zero",
+                                       coefs=CONFIG_COEFS, tune=tune)
+                    if i > 0:
+                        ms.append(rep.elapsed_ms)
+                res = (b if rep.result_in_b else a).clone()
+                v = len(ms) * T * h * n * n / (sum(ms) / 1e3) / 1e9
+                row[name] = {"gcells_per_s": round(v, 1), "ms_per_solve": round(sum(ms) / len(ms), 3),
+                             "us_per_step": round(sum(ms) / len(ms) / T * 1e3, 2), "grid": rep.grid[0],
+                             "steps_per_pass": rep.reserved[0]}
+                results[name] = res
+            row["bitwise_equal"] = all(torch.equal(results[k], results["per_step"]) for k in results)
+            best = max(("resident", "temporal_blocking", "per_step"), key=lambda k: row[k]["gcells_per_s"])
+            row["fastest"] = best
+            rows.append(row)
+            del a, b, init, results
+            torch.cuda.empty_cache()
+    # The resident kernel's step cost against its grid size: G workgroups of
+    # eight 62 x 8 x 6 FP64 blocks each (a 6 x 64 x 62G domain), so the work
+    # per wave is fixed and the growth is the grid barrier's.
+    scaling = []
+    for G in (1, 8, 64, 128, 256):
+        nz, ny, nx = 6, 64, 62 * G
+        a = torch.zeros(nz * ny * nx, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        ms = []
+        for i in range(3):
+            rep = P.j3d_device(a, b, nz, ny, nx, T, dtype="f64", method="persistent", boundary="zero",
+                               coefs=CONFIG_COEFS, tune=[0, 0, 0, 0, 5])
+            ms.append(rep.elapsed_ms)
+        scaling.append({"workgroups": rep.grid[0], "us_per_step": round(min(ms) / T * 1e3, 2)})
+        del a, b
+    print(json.dumps({"workload": "j3d-small: the 7-point solve at the getminHeight domain (zero halo, c=0.4 / 6x0.1)",
+                      "unit": "GCells/s", "rows": rows, "resident_step_vs_grid": scaling}), flush=True)
 
 
 def run_j3d_tune(args):
@@ -911,7 +976,7 @@ def main():
         sys.exit(1)
     {"j3d": run_j3d, "j3d-storage": run_j3d_storage, "j3d-shapes": run_j3d_shapes, "j2d-storage": run_j2d_storage,
      "batch2d": run_batch2d, "cg": run_cg, "cg-counters": run_cg_counters,
-     "j3d-tune": run_j3d_tune}[args.workload](args)
+     "j3d-tune": run_j3d_tune, "j3d-small": run_j3d_small}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -106,7 +106,9 @@ typedef struct perks_stencil_opts {
      *           with nearest rounding; perks_j3d_autotune measures every choice)
      *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel),
      *           4 = two columns per lane (FP32/FP16/BF16, even widths) or, for FP64,
-     *           8 waves x 5 rows.
+     *           8 waves x 5 rows; 5 = resident tiles (FP64/FP32): the whole domain held
+     *           in VGPRs across every step, one grid barrier per step, halos only through
+     *           memory; PERKS_ERR_UNSUPPORTED_SIZE above perks_j3d_min_height's height.
      *           3D other shapes: 0 = default (2), 2 = register streaming, 3 = LDS planes.
      *           2D: 0 = default (4 where it applies, else 2), 1 = LDS tile 128x64 (64x64 FP64),
      *           2 = row streaming through registers, one column per lane, 3 = LDS tile 128x32
@@ -187,7 +189,10 @@ int perks_shape_info(int shape, int32_t* dims, int32_t* radius, int32_t* ngroups
 /* ---- planning (host only when caps != NULL) ------------------------------ */
 int perks_j3d_plan(int dtype, int height, int width_y, int width_x, const perks_stencil_opts* opts,
                    const perks_device_caps* caps, int occupancy_hint, perks_plan_info* out);
-/* j3d_iterative(..., getminHeight=true): smallest height that caches fully. */
+/* j3d_iterative(..., getminHeight=true): the largest height whose domain the
+ * persistent kernel holds on chip for the whole solve (7-point star, FP64/FP32:
+ * the resident-tile variant, opts.tune[4] = 5; otherwise the height whose
+ * field fits the chip's LDS + VGPRs). */
 int perks_j3d_min_height(int dtype, int width_y, int width_x, const perks_stencil_opts* opts);
 /* getMinWidthY<REAL>(width_x, width_y, bdimx, isDoubleTile), j3d.cu:69-123. */
 int perks_get_min_width_y(int dtype, int width_x, int width_y, int bdimx, int double_tile);

@@ -43,6 +43,7 @@
 #include "api_common.h"
 #include "planner.h"
 #include "j3d_persistent_kernel.h"
+#include "j3d_resident.h"
 
 namespace perks {
 
@@ -128,6 +129,11 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
     if (o->shape != PERKS_SHAPE_3D_STAR_R1) return PERKS_ERR_UNSUPPORTED;
     if ((long long)ny * nx * elem_bytes(dtype) >= (long long)kOob) return PERKS_ERR_UNSUPPORTED_SIZE;
     if (occupancy <= 0) return PERKS_ERR_NOT_LAUNCHABLE;
+    if (o->tune[4] == kResidentVariant) {
+        int occ = std::min(occupancy, std::max(1, kCoopMaxWavesPerCu / 8));
+        if (o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
+        return plan_resident(dtype, nz, ny, nx, caps, occ, out);
+    }
     const Variant v = variant_shape(dtype, variant_of(o));
     if (v.pair && (nx & 1)) return PERKS_ERR_UNSUPPORTED;  // two columns per lane need an even width
     int occ = std::min(occupancy, std::max(1, kCoopMaxWavesPerCu / v.nw));
@@ -160,10 +166,12 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
 
 // The reference's getminHeight (j3d.cu:598,612) is the domain height at which
 // its persistent grid's register and shared-memory caches hold every plane.
-// Here the kernels stream planes and keep only ring buffers on chip, so the
-// equivalent figure is the height whose field fits in the chip's aggregate
-// on-chip storage (LDS plus VGPRs of every CU): the size below which a
-// persistent kernel could keep the whole domain resident.
+// For the 7-point star in FP64/FP32 that is the height the resident-tile
+// kernel (j3d_resident.hip) holds: as many layers of its blocks as the grid's
+// waves cover.  Other shapes and 16-bit storage stream planes and keep only
+// ring buffers on chip; for them the figure is the height whose field fits
+// the chip's aggregate on-chip storage (LDS plus VGPRs of every CU), the size
+// below which a persistent kernel could keep the whole domain resident.
 int min_height_caps(int dtype, int ny, int nx, const perks_device_caps* caps)
 {
     if (ny <= 0 || nx <= 0 || !caps) return PERKS_ERR_INVALID_ARG;
@@ -176,11 +184,22 @@ int min_height_caps(int dtype, int ny, int nx, const perks_device_caps* caps)
     return (int)std::min<long long>(h, 0x7fffffffLL);
 }
 
-int min_height_persistent(int dtype, int ny, int nx, const perks_stencil_opts*)
+int min_height_persistent(int dtype, int ny, int nx, const perks_stencil_opts* o)
 {
     perks_device_caps caps;
     const int rc = device_caps_cached(&caps);
     if (rc != PERKS_OK) return rc;
+    const bool star = !o || (o->shape == PERKS_SHAPE_3D_STAR_R1 && o->tune[5] != 1);
+    if (star && (dtype == PERKS_F64 || dtype == PERKS_F32)) {
+        if (ny <= 0 || nx <= 0) return PERKS_ERR_INVALID_ARG;
+        const int bc = o ? o->boundary : PERKS_BC_CLAMP;
+        int occ = 0;
+        const int r = resident_occupancy(dtype, bc, false, &occ);
+        if (r != PERKS_OK) return r;
+        occ = std::min(occ, std::max(1, kCoopMaxWavesPerCu / 8));
+        if (o && o->blkpsm > 0 && o->blkpsm < occ) occ = o->blkpsm;
+        return resident_height(dtype, ny, nx, &caps, occ);
+    }
     return min_height_caps(dtype, ny, nx, &caps);
 }
 
@@ -209,6 +228,10 @@ int run_persistent_fmt(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, 
                        const perks_stencil_opts* o, hipStream_t stream, perks_run_report* rep)
 {
     if (o->shape != PERKS_SHAPE_3D_STAR_R1) return PERKS_ERR_UNSUPPORTED;
+    if (o->tune[4] == kResidentVariant) {
+        if (o->tune[6] == 1) return PERKS_ERR_UNSUPPORTED;  // the contracted update is built for the ring kernel only
+        return run_resident(dtype, d_a, d_b, nz, ny, nx, steps, cf, o, stream, rep);
+    }
     perks_device_caps caps;
     int rc = device_caps_cached(&caps);
     if (rc != PERKS_OK) return rc;

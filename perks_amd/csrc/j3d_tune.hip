@@ -24,6 +24,7 @@
 #include "j3d_general.h"
 #include "j3d_kernels.h"
 #include "j3d_persistent.h"
+#include "j3d_resident.h"
 #include "planner.h"
 
 namespace perks {
@@ -61,6 +62,7 @@ std::vector<Candidate> candidates(int dtype, const perks_stencil_opts* o)
         for (int v = 0; v < nv; ++v) {
             for (int l = 1; l <= 4; ++l) c.push_back({PERKS_METHOD_PERSISTENT, v, l});
         }
+        c.push_back({PERKS_METHOD_PERSISTENT, kResidentVariant, 1});  // resident tiles: one step per barrier
         return c;
     }
     for (int k : {2, 3}) {  // register streaming, LDS planes
@@ -91,6 +93,9 @@ int describe(int dtype, int nz, int ny, int nx, const perks_stencil_opts* base, 
                                        : j3d_star1_plain_fn(dtype, o.boundary, has_div, o.tune[1] != 2);
             out->tile_x = elem_size(dtype) == 2 ? 128 : 64;
             out->tile_y = 32;
+        } else if (c.variant == kResidentVariant) {
+            if (o.tune[6] == 1) return out->status = PERKS_ERR_UNSUPPORTED;  // contracted: ring kernel only
+            fn = j3d_resident_fn(dtype, o.boundary, has_div, &block);
         } else {
             fn = j3d_persistent_fn(dtype, o.boundary, has_div, c.variant, sr, o.tune[6] == 1, &block);
         }

@@ -6,6 +6,8 @@ itself pinned to the reference gold in test_oracle.py).
 """
 import sys
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -158,15 +160,22 @@ def test_matches_committed_interior_gold(gpu, tag, method):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_min_height_is_the_on_chip_capacity(gpu, dtype):
-    """getminHeight / getMinWidthY: the height whose field fits the chip's LDS + VGPRs."""
-    caps = P.query_device()
-    onchip = caps.cu_count * (caps.lds_per_cu + caps.vgprs_per_simd * caps.simds_per_cu * caps.warp_size * 4)
+def test_min_height_is_the_resident_capacity(gpu, dtype):
+    """getminHeight / getMinWidthY: the largest height whose domain the
+    resident-tile kernel holds (blocks of 62 columns x 8 rows x 6 (FP64) or
+    12 (FP32) planes, one per wave); it runs there and refuses one more layer."""
     wy, wx = 512, 512
-    plane = wy * wx * np.dtype(dtype).itemsize
     h = P.j3d_iterative(np.empty(0, dtype), 0, wy, wx, None, getminHeight=True)
-    assert h * plane <= onchip < (h + 1) * plane
+    bz = 6 if dtype == np.float64 else 12
+    caps = P.query_device()
+    assert h > 0 and h % bz == 0
+    assert h // bz * -(-wx // 62) * -(-wy // 8) <= caps.cu_count * 8
     lib = P.load()
     dt = 0 if dtype == np.float64 else 1
     assert lib.perks_get_min_width_y(dt, wx, wy, 256, 0) == h
-    assert lib.perks_get_min_width_y(dt, 8192, 8192, 256, 0) == -4  # not even three planes fit
+    assert lib.perks_get_min_width_y(dt, 8192, 8192, 256, 0) == -4  # not one layer of blocks fits
+    # the 27-point box keeps the on-chip-capacity figure
+    o = P.make_opts(shape="3d27pt")
+    onchip = caps.cu_count * (caps.lds_per_cu + caps.vgprs_per_simd * caps.simds_per_cu * caps.warp_size * 4)
+    h27 = lib.perks_j3d_min_height(dt, wy, wx, C.byref(o))
+    assert h27 * wy * wx * np.dtype(dtype).itemsize <= onchip < (h27 + 1) * wy * wx * np.dtype(dtype).itemsize
