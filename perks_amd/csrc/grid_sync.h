@@ -43,6 +43,19 @@ __device__ __forceinline__ unsigned int* gs_count(GridSync* gs, int g)
     return (unsigned int*)(gs->w + kSyncStride * (g + 1));
 }
 
+// Group layout of a grid of nblocks workgroups: ng = min(nblocks, kSyncGroups)
+// groups, workgroup w in group w % ng, so group g has
+// ceil((nblocks - g) / ng) members.  Host-callable so the arithmetic can be
+// checked off the device.
+__host__ __device__ constexpr unsigned int gs_groups(unsigned int nblocks)
+{
+    return nblocks < (unsigned int)kSyncGroups ? nblocks : (unsigned int)kSyncGroups;
+}
+__host__ __device__ constexpr unsigned int gs_members(unsigned int nblocks, unsigned int g)
+{
+    return (nblocks - g + gs_groups(nblocks) - 1) / gs_groups(nblocks);
+}
+
 // Waits until this thread's outstanding memory operations (here: an atomic
 // exchange) have completed, and keeps the compiler from moving memory
 // operations across the wait.
@@ -71,9 +84,9 @@ __device__ __forceinline__ bool grid_sync(GridSync* gs, unsigned int nblocks, un
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        const unsigned int ng = nblocks < (unsigned)kSyncGroups ? nblocks : (unsigned)kSyncGroups;
+        const unsigned int ng = gs_groups(nblocks);
         const int g = (int)(blockIdx.x % ng);
-        const unsigned int members = (nblocks - (unsigned)g + ng - 1) / ng;
+        const unsigned int members = gs_members(nblocks, (unsigned int)g);
         unsigned int* gen = gs_gen(gs, g);
         // The release word is read before arriving (it cannot change without this
         // arrival); the release fence below also waits for this load.

@@ -559,13 +559,37 @@ struct perks_batch_plan {
     perks::GridSync* gs = nullptr;  // the plan's own barrier block (grid_sync_alloc)
     std::mutex run_mu;              // runs of one plan share that block: one at a time
     hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = nullptr;        // persistent method: the kernel node marked cooperative
+    hipGraphExec_t exec_plain = nullptr;  // the same graph as an ordinary launch (fallback)
     int tuned = 0;
     double tuned_gcells = 0.0;
 };
 
 namespace perks {
 namespace {
+
+// A launch captured from a stream loses its cooperative property: the kernel
+// node of a captured hipLaunchCooperativeKernel reads its cooperative
+// attribute as 0 (HIP 7.x, MI355X).  Marking the node cooperative makes a
+// replay a cooperative launch again, so the runtime, not the caller,
+// guarantees that the persistent grid is co-resident.
+int mark_cooperative(hipGraph_t g)
+{
+    size_t n = 0;
+    PERKS_HIP_TRY(hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    PERKS_HIP_TRY(hipGraphGetNodes(g, nodes.data(), &n));
+    for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType t;
+        PERKS_HIP_TRY(hipGraphNodeGetType(node, &t));
+        if (t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeAttrValue v;
+        std::memset(&v, 0, sizeof(v));
+        v.cooperative = 1;
+        PERKS_HIP_TRY(hipGraphKernelNodeSetAttribute(node, hipKernelNodeAttributeCooperative, &v));
+    }
+    return PERKS_OK;
+}
 
 // Times one configuration on scratch buffers: every grid runs min(steps, cap)
 // steps.  Returns cells per second (0 when the configuration does not apply).
@@ -709,7 +733,16 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
             if (rc == PERKS_OK && (erc != PERKS_OK || ec != hipSuccess)) rc = erc != PERKS_OK ? erc : PERKS_ERR_HIP;
             if (rc == PERKS_OK) {
                 p->graph = g;
-                if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) rc = PERKS_ERR_HIP;
+                if (hipGraphInstantiate(&p->exec_plain, g, nullptr, nullptr, 0) != hipSuccess) rc = PERKS_ERR_HIP;
+                if (rc == PERKS_OK && o->method != PERKS_METHOD_PLAIN) {
+                    // the persistent grid: a cooperative replay, the ordinary one kept as the fallback
+                    rc = mark_cooperative(g);
+                    if (rc == PERKS_OK && hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess)
+                        rc = PERKS_ERR_HIP;
+                } else {
+                    p->exec = p->exec_plain;
+                    p->exec_plain = nullptr;
+                }
             } else if (g) {
                 (void)hipGraphDestroy(g);
             }
@@ -718,6 +751,7 @@ int plan_batch(int dtype, const perks_grid2d* grids, int ngrids, const perks_ste
     }
     if (rc != PERKS_OK) {
         if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->exec_plain) (void)hipGraphExecDestroy(p->exec_plain);
         if (p->graph) (void)hipGraphDestroy(p->graph);
         (void)p->dw.release();
         (void)grid_sync_free(p->gs);
@@ -754,7 +788,17 @@ int perks_jacobi2d_batch_run(perks_batch_plan* p, void* stream_v, perks_run_repo
     PERKS_HIP_TRY(hipEventCreate(&ev0));
     PERKS_HIP_TRY(hipEventCreate(&ev1));
     PERKS_HIP_TRY(hipEventRecord(ev0, stream));
-    if (p->exec) PERKS_HIP_TRY(hipGraphLaunch(p->exec, stream));
+    if (p->exec) {
+        hipError_t e = hipGraphLaunch(p->exec, stream);
+        if (e == hipErrorIllegalState && p->exec_plain) {
+            // another process holds the device's cooperative queue: the
+            // ordinary replay (co-residency then rests on an otherwise idle
+            // device; the barrier's bounded wait reports a grid that is not)
+            (void)hipGetLastError();
+            e = hipGraphLaunch(p->exec_plain, stream);
+        }
+        PERKS_HIP_TRY(e);
+    }
     PERKS_HIP_TRY(hipEventRecord(ev1, stream));
     PERKS_HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0.f;
@@ -791,6 +835,7 @@ void perks_jacobi2d_batch_plan_destroy(perks_batch_plan* p)
 {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->exec_plain) (void)hipGraphExecDestroy(p->exec_plain);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     (void)p->dw.release();
     (void)perks::grid_sync_free(p->gs);
