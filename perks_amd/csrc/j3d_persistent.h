@@ -19,9 +19,14 @@ int release_persistent_scratch();
 
 // Number of tile variants (opts.tune[4]) of the persistent 7-point kernel.
 int j3d_persistent_variants();
-// The persistent kernel of tile variant `var` (fused: the contracted update),
-// and its workgroup size in *block, for attribute and occupancy queries;
-// nullptr when not built.
-void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, bool fused, int* block);
+// The form of the 7-point update a persistent run uses (j3dp::kUpd*): the
+// contracted one when opts.tune[6] = 1; the shared-product one (the gold's
+// bits) when the six neighbour weights are one positive value in the
+// arithmetic type of FP64/FP32 storage; the gold's otherwise.
+int j3d_persistent_update(int dtype, const perks_stencil_opts* o, const KCoefs& cf);
+// The persistent kernel of tile variant `var` and update form `up`, and its
+// workgroup size in *block, for attribute and occupancy queries; nullptr when
+// not built.
+void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, int up, int* block);
 
 }  // namespace perks

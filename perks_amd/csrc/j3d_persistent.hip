@@ -55,13 +55,13 @@ using j3dp::PersistArgs;
 using j3dp::Variant;
 using j3dp::variant_shape;
 
-void* pick_kernel(int fmt, int bc, bool has_div, int var, bool sr, bool fused)
+void* pick_kernel(int fmt, int bc, bool has_div, int var, bool sr, int up)
 {
     switch (fmt) {
-    case PERKS_F64: return j3d_persistent_kernel_f64(bc, has_div, var, sr, fused);
-    case PERKS_F32: return j3d_persistent_kernel_f32(bc, has_div, var, sr, fused);
-    case PERKS_F16: return j3d_persistent_kernel_f16(bc, has_div, var, sr, fused);
-    case PERKS_BF16: return j3d_persistent_kernel_bf16(bc, has_div, var, sr, fused);
+    case PERKS_F64: return j3d_persistent_kernel_f64(bc, has_div, var, sr, up);
+    case PERKS_F32: return j3d_persistent_kernel_f32(bc, has_div, var, sr, up);
+    case PERKS_F16: return j3d_persistent_kernel_f16(bc, has_div, var, sr, up);
+    case PERKS_BF16: return j3d_persistent_kernel_bf16(bc, has_div, var, sr, up);
     default: return nullptr;
     }
 }
@@ -213,11 +213,27 @@ int min_width_y_persistent(int dtype, int nx, int ny, int, int)
 
 int j3d_persistent_variants() { return kNumVariants; }
 
-void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, bool fused, int* block)
+int j3d_persistent_update(int dtype, const perks_stencil_opts* o, const KCoefs& cf)
+{
+    if (o->tune[6] == 1) return j3dp::kUpdFused;
+    bool same = false;
+    if (dtype == PERKS_F64) {
+        const double k = cf.c[0];
+        same = k > 0.0;
+        for (int i = 1; i < 6; ++i) same = same && cf.c[i] == k;
+    } else if (dtype == PERKS_F32) {
+        const float k = (float)cf.c[0];
+        same = k > 0.0f;
+        for (int i = 1; i < 6; ++i) same = same && (float)cf.c[i] == k;
+    }
+    return same ? j3dp::kUpdShared : j3dp::kUpdGold;
+}
+
+void* j3d_persistent_fn(int dtype, int bc, bool has_div, int var, bool sr, int up, int* block)
 {
     if (var < 0 || var >= kNumVariants) return nullptr;
     if (block) *block = variant_shape(dtype, var).nw * 64;
-    return pick_kernel(dtype, bc, has_div, var, sr, fused);
+    return pick_kernel(dtype, bc, has_div, var, sr, up);
 }
 
 // ---- launch -----------------------------------------------------------------
@@ -240,7 +256,8 @@ int run_persistent_fmt(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, 
     const int var = variant_of(o);
     // tune[6] = 1: the contracted update (FP64/FP32 without a divisor; the
     // kernel pickers return nullptr for the rest)
-    void* fn = pick_kernel(dtype, o->boundary, has_div, var, o->rounding == PERKS_ROUND_STOCHASTIC, o->tune[6] == 1);
+    void* fn = pick_kernel(dtype, o->boundary, has_div, var, o->rounding == PERKS_ROUND_STOCHASTIC,
+                           j3d_persistent_update(dtype, o, cf));
     if (!fn) return PERKS_ERR_UNSUPPORTED;
     int occ = 0;
     PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn),

@@ -2,8 +2,8 @@
 #include "j3d_persistent_kernel.h"
 
 namespace perks {
-void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant, bool sr, bool fused)
+void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant, bool sr, int up)
 {
-    return j3dp::pick_kernel<PERKS_F64>(bc, has_div, variant, sr, fused);
+    return j3dp::pick_kernel<PERKS_F64>(bc, has_div, variant, sr, up);
 }
 }  // namespace perks

@@ -16,6 +16,18 @@ namespace j3dp {
 
 constexpr int kMaxLevels = 4;  // time steps per pass the kernel is built for
 
+// Forms of the 7-point update (template argument UP):
+//   kUpdGold    star1_update: every product and sum rounded as the gold writes them
+//   kUpdFused   star1_update_fused: products after the first fused into the sum
+//   kUpdShared  the gold's bits with fewer multiplies, for six equal positive
+//               neighbour weights k: every neighbour term is k times a cell
+//               value, so k*x is formed once per cell of a row (and of the
+//               rows just outside it) and shifted or indexed to every term
+//               that reads x, instead of once per term
+constexpr int kUpdGold = 0;
+constexpr int kUpdFused = 1;
+constexpr int kUpdShared = 2;
+
 struct PersistArgs {
     int nz, ny, nx;
     int steps;
@@ -225,7 +237,7 @@ __device__ __forceinline__ double read_lane_t<double>(double v, int l)
 // level 0 through the load, higher levels through side_faces), so waves at
 // an x face take the update without boundary tests.  Waves at a y face and
 // the planes at z faces apply the rule by selects (compute<true>).
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, int UP>
 struct March {
     using E = Elem<FMT>;
     using S = typename E::S;
@@ -278,6 +290,44 @@ struct March {
     {
         const bool zlo = EDGE && p == 0;
         const bool zhi = EDGE && p == nz - 1;
+        if constexpr (UP == kUpdShared) {
+            // q[j] = k * (row j-1 of the centre plane): the north, south, east
+            // and west terms of the neighbouring rows and lanes.  A lane with
+            // no neighbour gets +0 from the shift, which is k * 0 for k > 0,
+            // as in the gold form.
+            const T kn = k[0];
+            T q[NY + 2];
+            q[0] = kn * c_lo;
+#pragma unroll
+            for (int r = 0; r < NY; ++r) q[r + 1] = kn * c[o + r];
+            q[NY + 1] = kn * c_hi;
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                const T cc = c[o + r];
+                T te = dpp_shift0<kDppWaveShl1>(q[r + 1]);
+                T tw = dpp_shift0<kDppWaveShr1>(q[r + 1]);
+                T tn = q[r + 2], ts = q[r];
+                T tt = kn * u[o + r], tb = kn * b[o + r];
+                if (EDGE) {
+                    const T bq = BC == PERKS_BC_CLAMP ? q[r + 1] : (T)0;  // k * (the boundary value)
+                    tw = xlo ? bq : tw;
+                    te = xhi ? bq : te;
+                    tn = yhi[r] ? bq : tn;
+                    ts = ylo[r] ? bq : ts;
+                    tt = zhi ? bq : tt;
+                    tb = zlo ? bq : tb;
+                }
+                T acc = te + tw;
+                acc = acc + tn;
+                acc = acc + ts;
+                acc = acc + tt;
+                acc = acc + tb;
+                acc = acc + k[6] * cc;
+                if (HAS_DIV) acc = acc / k[7];
+                out[r] = E::template qr<SR>(acc, (uint32_t)p * plane32 + cbase + (uint32_t)(r * nx), key[lv]);
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < NY; ++r) {
             const T cc = c[o + r];
@@ -295,8 +345,8 @@ struct March {
                 t = zhi ? bv : t;
                 bt = zlo ? bv : bt;
             }
-            const T v = FU ? star1_update_fused<T, HAS_DIV>(k, e, w, n, s, t, bt, cc)
-                           : star1_update<T, HAS_DIV>(k, e, w, n, s, t, bt, cc);
+            const T v = UP == kUpdFused ? star1_update_fused<T, HAS_DIV>(k, e, w, n, s, t, bt, cc)
+                                        : star1_update<T, HAS_DIV>(k, e, w, n, s, t, bt, cc);
             out[r] = E::template qr<SR>(v, (uint32_t)p * plane32 + cbase + (uint32_t)(r * nx), key[lv]);
         }
     }
@@ -532,7 +582,7 @@ struct Elem2<PERKS_BF16> : Elem2x16<PERKS_BF16> {};
 // columns (stores stay on whole pairs; the grid width must be even).  Level
 // structure, the one-iteration-late row exchange and the face handling are
 // those of March.
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, int UP>
 struct March2 {
     using E = Elem2<FMT>;
     using S = typename Elem<FMT>::S;
@@ -613,10 +663,10 @@ struct March2 {
                 ba = zlo ? bva : ba;
                 bb = zlo ? bvb : bb;
             }
-            T va = FU ? star1_update_fused<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca)
-                      : star1_update<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca);
-            T vb = FU ? star1_update_fused<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb)
-                      : star1_update<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb);
+            T va = UP == kUpdFused ? star1_update_fused<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca)
+                                   : star1_update<T, HAS_DIV>(k, ea, wa, na, sa, ta, ba, ca);
+            T vb = UP == kUpdFused ? star1_update_fused<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb)
+                                   : star1_update<T, HAS_DIV>(k, eb, wb, nb, sb, tb, bb, cb);
             E::template qr<SR>(va, vb, (uint32_t)p * plane32 + cbase + (uint32_t)(r * nx), key[lv]);
             out[r][0] = va;
             out[r][1] = vb;
@@ -792,7 +842,7 @@ struct March2 {
     }
 };
 
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, int UP>
 __device__ __forceinline__ void run_pass2(const typename Elem<FMT>::S* src, typename Elem<FMT>::S* dst, float2* xch,
                                           const PersistArgs& a, const float* k, int lane, int wave, int step0)
 {
@@ -804,7 +854,7 @@ __device__ __forceinline__ void run_pass2(const typename Elem<FMT>::S* src, type
     for (int item = xcd_tile_id(blockIdx.x, G); item < nitems; item += G) {
         int t, zs, ze;
         item_piece(item, ntx * nty, a.nzs[L], a.nz, t, zs, ze);
-        March2<FMT, BC, NY, NW, P, L, HAS_DIV, SR, FU> m;
+        March2<FMT, BC, NY, NW, P, L, HAS_DIV, SR, UP> m;
         m.src = src;
         m.dst = dst;
         m.xch = xch;
@@ -826,7 +876,7 @@ __device__ __forceinline__ void run_pass2(const typename Elem<FMT>::S* src, type
     }
 }
 
-template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR, bool FU>
+template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR, int UP>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE)))
 j3d_star1_persistent2_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S* buf_b, PersistArgs a, KCoefs cf)
 {
@@ -843,10 +893,10 @@ j3d_star1_persistent2_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S
     for (int pass = 0; pass < npass; ++pass) {
         const int lv = (pass == npass - 1) ? a.steps - pass * L : L;
         switch (lv) {
-        case 1: run_pass2<FMT, BC, NY, NW, P, 1, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 2: run_pass2<FMT, BC, NY, NW, P, 2, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 3: run_pass2<FMT, BC, NY, NW, P, 3, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        default: run_pass2<FMT, BC, NY, NW, P, 4, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 1: run_pass2<FMT, BC, NY, NW, P, 1, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 2: run_pass2<FMT, BC, NY, NW, P, 2, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 3: run_pass2<FMT, BC, NY, NW, P, 3, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        default: run_pass2<FMT, BC, NY, NW, P, 4, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
         }
         S* t = src;
         src = dst;
@@ -855,7 +905,7 @@ j3d_star1_persistent2_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S
     }
 }
 
-template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, bool FU>
+template <int FMT, int BC, int NY, int NW, int P, int L, bool HAS_DIV, bool SR, int UP>
 __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typename Elem<FMT>::S* dst,
                                          typename Elem<FMT>::T* xch, const PersistArgs& a,
                                          const typename Elem<FMT>::T* k, int lane, int wave, int step0)
@@ -867,7 +917,7 @@ __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typen
     for (int item = xcd_tile_id(blockIdx.x, G); item < nitems; item += G) {
         int t, zs, ze;
         item_piece(item, ntx * nty, a.nzs[L], a.nz, t, zs, ze);
-        March<FMT, BC, NY, NW, P, L, HAS_DIV, SR, FU> m;
+        March<FMT, BC, NY, NW, P, L, HAS_DIV, SR, UP> m;
         m.src = src;
         m.dst = dst;
         m.xch = xch;
@@ -889,7 +939,7 @@ __device__ __forceinline__ void run_pass(const typename Elem<FMT>::S* src, typen
     }
 }
 
-template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR, bool FU>
+template <int FMT, int BC, int NY, int NW, int P, int WPE, bool HAS_DIV, bool SR, int UP>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE)))
 j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S* buf_b, PersistArgs a, KCoefs cf)
 {
@@ -907,10 +957,10 @@ j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S*
     for (int pass = 0; pass < npass; ++pass) {
         const int lv = (pass == npass - 1) ? a.steps - pass * L : L;
         switch (lv) {
-        case 1: run_pass<FMT, BC, NY, NW, P, 1, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 2: run_pass<FMT, BC, NY, NW, P, 2, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        case 3: run_pass<FMT, BC, NY, NW, P, 3, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
-        default: run_pass<FMT, BC, NY, NW, P, 4, HAS_DIV, SR, FU>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 1: run_pass<FMT, BC, NY, NW, P, 1, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 2: run_pass<FMT, BC, NY, NW, P, 2, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        case 3: run_pass<FMT, BC, NY, NW, P, 3, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
+        default: run_pass<FMT, BC, NY, NW, P, 4, HAS_DIV, SR, UP>(src, dst, xch, a, k, lane, wave, pass * L); break;
         }
         S* t = src;
         src = dst;
@@ -947,63 +997,67 @@ __host__ __device__ constexpr Variant variant_shape(int fmt, int v)
     return fmt == PERKS_F64 ? kVarF64[v] : kVarF32[v];
 }
 
-template <int FMT, int BC, bool HAS_DIV, int V, bool SR, bool FU>
+template <int FMT, int BC, bool HAS_DIV, int V, bool SR, int UP>
 void* kernel_ptr()
 {
     constexpr Variant v = variant_shape(FMT, V);
+    // the two-column form runs the shared-product weights in the gold form (the same bits)
+    constexpr int UP2 = UP == kUpdShared ? kUpdGold : UP;
     if constexpr (v.pair != 0 && FMT != PERKS_F64)
-        return reinterpret_cast<void*>(&j3d_star1_persistent2_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR, FU>);
+        return reinterpret_cast<void*>(&j3d_star1_persistent2_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR, UP2>);
     else
-        return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR, FU>);
+        return reinterpret_cast<void*>(&j3d_star1_persistent_kernel<FMT, BC, v.ny, v.nw, v.p, v.wpe, HAS_DIV, SR, UP>);
 }
 
-template <int FMT, int BC, bool HAS_DIV, bool SR, bool FU>
+template <int FMT, int BC, bool HAS_DIV, bool SR, int UP>
 void* pick_variant(int var)
 {
     switch (var) {
-    case 1: return kernel_ptr<FMT, BC, HAS_DIV, 1, SR, FU>();
-    case 2: return kernel_ptr<FMT, BC, HAS_DIV, 2, SR, FU>();
-    case 3: return kernel_ptr<FMT, BC, HAS_DIV, 3, SR, FU>();
-    case 4: return kernel_ptr<FMT, BC, HAS_DIV, 4, SR, FU>();
-    default: return kernel_ptr<FMT, BC, HAS_DIV, 0, SR, FU>();
+    case 1: return kernel_ptr<FMT, BC, HAS_DIV, 1, SR, UP>();
+    case 2: return kernel_ptr<FMT, BC, HAS_DIV, 2, SR, UP>();
+    case 3: return kernel_ptr<FMT, BC, HAS_DIV, 3, SR, UP>();
+    case 4: return kernel_ptr<FMT, BC, HAS_DIV, 4, SR, UP>();
+    default: return kernel_ptr<FMT, BC, HAS_DIV, 0, SR, UP>();
     }
 }
 
-template <int FMT, bool SR>
+template <int FMT, bool SR, int UP>
 void* pick_bc(int bc, bool has_div, int var)
 {
     if (bc == PERKS_BC_ZERO)
-        return has_div ? pick_variant<FMT, PERKS_BC_ZERO, true, SR, false>(var)
-                       : pick_variant<FMT, PERKS_BC_ZERO, false, SR, false>(var);
-    return has_div ? pick_variant<FMT, PERKS_BC_CLAMP, true, SR, false>(var)
-                   : pick_variant<FMT, PERKS_BC_CLAMP, false, SR, false>(var);
+        return has_div ? pick_variant<FMT, PERKS_BC_ZERO, true, SR, UP>(var)
+                       : pick_variant<FMT, PERKS_BC_ZERO, false, SR, UP>(var);
+    return has_div ? pick_variant<FMT, PERKS_BC_CLAMP, true, SR, UP>(var)
+                   : pick_variant<FMT, PERKS_BC_CLAMP, false, SR, UP>(var);
 }
 
 // sr: stochastic rounding (16-bit storage only; FP64/FP32 ignore it).
-// fused: the contracted update (FP64/FP32 storage without a divisor only;
-// nullptr otherwise).
+// up: the update form (kUpd*); the fused form is built for FP64/FP32 storage
+// without a divisor, the shared-product form for FP64/FP32 storage (nullptr
+// otherwise).
 template <int FMT>
-void* pick_kernel(int bc, bool has_div, int var, bool sr, bool fused)
+void* pick_kernel(int bc, bool has_div, int var, bool sr, int up)
 {
     if constexpr (FMT == PERKS_F64 || FMT == PERKS_F32) {
-        if (fused) {
+        if (up == kUpdFused) {
             if (has_div) return nullptr;
-            return bc == PERKS_BC_ZERO ? pick_variant<FMT, PERKS_BC_ZERO, false, false, true>(var)
-                                       : pick_variant<FMT, PERKS_BC_CLAMP, false, false, true>(var);
+            return bc == PERKS_BC_ZERO ? pick_variant<FMT, PERKS_BC_ZERO, false, false, kUpdFused>(var)
+                                       : pick_variant<FMT, PERKS_BC_CLAMP, false, false, kUpdFused>(var);
         }
+        if (up == kUpdShared) return pick_bc<FMT, false, kUpdShared>(bc, has_div, var);
     } else {
-        if (fused) return nullptr;
-        if (sr) return pick_bc<FMT, true>(bc, has_div, var);
+        if (up != kUpdGold) return nullptr;
+        if (sr) return pick_bc<FMT, true, kUpdGold>(bc, has_div, var);
     }
-    return pick_bc<FMT, false>(bc, has_div, var);
+    return pick_bc<FMT, false, kUpdGold>(bc, has_div, var);
 }
 
 }  // namespace j3dp
 
 // Kernel entry points per storage format (j3d_persistent_<fmt>.hip).
-void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant, bool sr, bool fused);
-void* j3d_persistent_kernel_f32(int bc, bool has_div, int variant, bool sr, bool fused);
-void* j3d_persistent_kernel_f16(int bc, bool has_div, int variant, bool sr, bool fused);
-void* j3d_persistent_kernel_bf16(int bc, bool has_div, int variant, bool sr, bool fused);
+void* j3d_persistent_kernel_f64(int bc, bool has_div, int variant, bool sr, int up);
+void* j3d_persistent_kernel_f32(int bc, bool has_div, int variant, bool sr, int up);
+void* j3d_persistent_kernel_f16(int bc, bool has_div, int variant, bool sr, int up);
+void* j3d_persistent_kernel_bf16(int bc, bool has_div, int variant, bool sr, int up);
 
 }  // namespace perks

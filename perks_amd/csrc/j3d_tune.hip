@@ -75,8 +75,9 @@ std::vector<Candidate> candidates(int dtype, const perks_stencil_opts* o)
 }
 
 int describe(int dtype, int nz, int ny, int nx, const perks_stencil_opts* base, const Candidate& c,
-             const perks_device_caps& caps, bool has_div, perks_kernel_config* out)
+             const perks_device_caps& caps, const KCoefs& cf, perks_kernel_config* out)
 {
+    const bool has_div = cf.divisor != 0.0;
     std::memset(out, 0, sizeof(*out));
     out->method = c.method;
     out->variant = c.variant;
@@ -97,7 +98,8 @@ int describe(int dtype, int nz, int ny, int nx, const perks_stencil_opts* base, 
             if (o.tune[6] == 1) return out->status = PERKS_ERR_UNSUPPORTED;  // contracted: ring kernel only
             fn = j3d_resident_fn(dtype, o.boundary, has_div, &block);
         } else {
-            fn = j3d_persistent_fn(dtype, o.boundary, has_div, c.variant, sr, o.tune[6] == 1, &block);
+            fn = j3d_persistent_fn(dtype, o.boundary, has_div, c.variant, sr, j3d_persistent_update(dtype, &o, cf),
+                                   &block);
         }
     } else {
         fn = j3g_fn(dtype, &o, &block, &lds);
@@ -164,7 +166,7 @@ int build_table(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, 
     *cands = candidates(dtype, o);
     rows->resize(cands->size());
     for (size_t i = 0; i < cands->size(); ++i) {
-        rc = describe(dtype, nz, ny, nx, o, (*cands)[i], caps, cf.divisor != 0.0, &(*rows)[i]);
+        rc = describe(dtype, nz, ny, nx, o, (*cands)[i], caps, cf, &(*rows)[i]);
         if (rc == PERKS_ERR_HIP) return rc;
     }
     return PERKS_OK;

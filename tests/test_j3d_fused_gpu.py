@@ -20,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 CONFIG_3D = [0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4]
 FUSED = 1
+RESIDENT = 5  # opts.tune[4]: the resident-tile variant
 
 
 def say(*a):
@@ -54,11 +55,12 @@ def test_fused_matches_oracle(gpu, dtype, variant, boundary):
 def test_fused_table_has_no_spills(gpu):
     for dtype in ("f64", "f32"):
         rows = P.j3d_kernel_table(64, 64, 64, dtype=dtype, shape="3d7pt", boundary="zero", tune=tune())
-        persistent = [r for r in rows if r["method"] == "persistent"]
+        persistent = [r for r in rows if r["method"] == "persistent" and r["variant"] != RESIDENT]
         assert len(persistent) == 4 * 5
         for r in persistent:
             assert r["spill_bytes"] == 0 and r["status"] == L.PERKS_OK, r
-        assert all(r["status"] == L.ERR_UNSUPPORTED for r in rows if r["method"] == "plain")
+        # the per-step kernel and the resident-tile one are built without it
+        assert all(r["status"] == L.ERR_UNSUPPORTED for r in rows if r not in persistent)
 
 
 def test_fused_refused_where_not_built(gpu):
@@ -86,8 +88,8 @@ def test_fused_refused_where_not_built(gpu):
 def test_fused_full_size_stays_near_gold(gpu, dtype):
     """512^3, 1000 steps, the benchmark's inputs: the contracted result against
     the uncontracted one (which the other tests pin to the gold bit for bit).
-    Measured: FP64 within 2e-13, FP32 within 1e-4 relative to the field's
-    largest value; bounds with margin."""
+    Measured: FP64 1.3e-14, FP32 9.4e-6 of the field's largest value; the
+    bounds leave margin."""
     torch = gpu
     n, T = 512, 1000
     tdt = torch.float64 if dtype == "f64" else torch.float32
