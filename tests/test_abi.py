@@ -104,11 +104,15 @@ def test_planner_is_host_only_and_refuses_like_the_reference():
 
 
 def test_planner_balances_items_over_workgroups():
-    # 512^3, 4 steps per pass, 56x26 valid tiles: 200 tiles; the z split must
-    # not leave a lone round for a handful of items.
+    # 512^3, 4 steps per pass, the clamp boundary: tiles of 64 x tile_y lanes
+    # store 56 columns and tile_y - 6 rows (tile_y - 3 at the y faces); the z
+    # split must not leave a lone round for a handful of items.
     rc, info = P.plan(512, 512, 512, caps=caps(), occupancy=1, method="persistent")
+    rows = info.tile_y
+    ntx = -(-512 // 56)
+    nty = 2 + -(-(512 - 2 * (rows - 3)) // (rows - 6))
     nzs = info.reserved[4]
-    items = 200 * nzs
+    items = ntx * nty * nzs
     rounds = -(-items // 256)
     assert items / (rounds * 256) > 0.9
 
