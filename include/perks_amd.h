@@ -109,7 +109,8 @@ typedef struct perks_stencil_opts {
      *           widths) or, for FP64, 8 waves x 4 rows; 5 = resident tiles (FP64/FP32): the whole domain held
      *           in VGPRs across every step, one grid barrier per step, halos only through
      *           memory; PERKS_ERR_UNSUPPORTED_SIZE above perks_j3d_min_height's height.
-     *           3D other shapes: 0 = default (2), 2 = register streaming, 3 = LDS planes.
+     *           3D other shapes: 0 = default, 2 = register streaming (64 x 8 tiles),
+     *           4 = register streaming (64 x 16 tiles), 3 = LDS planes.
      *           2D: 0 = default (4 where it applies, else 2), 1 = LDS tile 128x64 (64x64 FP64),
      *           2 = row streaming through registers, one column per lane, 3 = LDS tile 128x32
      *           (64x32 FP64), 4 = row streaming, two columns per lane (FP32/FP16/BF16 storage,

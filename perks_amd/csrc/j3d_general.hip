@@ -35,22 +35,42 @@ void* kernel_for(int dtype, int shape, int bc)
     }
 }
 
-void* stream_kernel_for(int dtype, int shape, int bc)
+void* stream_kernel_for(int dtype, int shape, int bc, int rows)
 {
     switch (dtype) {
-    case PERKS_F64: return j3s_kernel_f64(shape, bc);
-    case PERKS_F32: return j3s_kernel_f32(shape, bc);
-    case PERKS_F16: return j3s_kernel_f16(shape, bc);
-    case PERKS_BF16: return j3s_kernel_bf16(shape, bc);
+    case PERKS_F64: return j3s_kernel_f64(shape, bc, rows);
+    case PERKS_F32: return j3s_kernel_f32(shape, bc, rows);
+    case PERKS_F16: return j3s_kernel_f16(shape, bc, rows);
+    case PERKS_BF16: return j3s_kernel_bf16(shape, bc, rows);
     default: return nullptr;
     }
 }
 
+// The default tile height of the streaming kernel per storage format, radius
+// and method (see stream_rows).
+int default_stream_rows(int dtype, int radius, int method)
+{
+    (void)dtype;
+    (void)radius;
+    (void)method;
+    return j3s::kRows;
+}
+
 bool use_stream(const perks_stencil_opts* o, int radius)
 {
-    if (o->tune[4] == 2) return true;
+    if (o->tune[4] == 2 || o->tune[4] == 4) return true;
     (void)radius;
     return o->tune[4] != 3;
+}
+
+// Tile rows per wave of the streaming kernel: tune[4] = 4 the tall tile
+// (j3s::kRowsTall), 2 the short one (j3s::kRows); by default the tall tile
+// where it measured faster.
+int stream_rows(int dtype, int radius, const perks_stencil_opts* o)
+{
+    if (o->tune[4] == 4) return j3s::kRowsTall;
+    if (o->tune[4] == 2) return j3s::kRows;
+    return default_stream_rows(dtype, radius, o->method);
 }
 
 constexpr int kStreamItems = 16384;  // per-step launches: work items (waves) to aim for
@@ -122,12 +142,13 @@ int j3g_plan(int dtype, int nz, int ny, int nx, const perks_stencil_opts* o, con
     const int R = sd->radius;
     const int L = j3g_levels(dtype, o->shape, o);
     if (use_stream(o, R)) {
-        const int VW = 64 - 2 * R * L, VH = j3s::kRows - 2 * R * L;
+        const int rows = stream_rows(dtype, R, o);
+        const int VW = 64 - 2 * R * L, VH = rows - 2 * R * L;
         const long long tiles_xy = (long long)((nx + VW - 1) / VW) * ((ny + VH - 1) / VH);
         std::memset(out, 0, sizeof(*out));
         out->block = j3s::kWaves * 64;
         out->tile_x = 64;
-        out->tile_y = j3s::kRows;
+        out->tile_y = rows;
         out->reserved[0] = L;
         int nzs = 1;
         long long groups = 0;
@@ -184,7 +205,8 @@ void* j3g_fn(int dtype, const perks_stencil_opts* o, int* block, size_t* lds)
     const bool streaming = use_stream(o, sd->radius);
     if (block) *block = streaming ? j3s::kWaves * 64 : j3g::kThreads;
     if (lds) *lds = streaming ? 0 : lds_for(dtype, sd->radius, j3g_levels(dtype, o->shape, o));
-    return streaming ? stream_kernel_for(dtype, o->shape, o->boundary) : kernel_for(dtype, o->shape, o->boundary);
+    return streaming ? stream_kernel_for(dtype, o->shape, o->boundary, stream_rows(dtype, sd->radius, o))
+                     : kernel_for(dtype, o->shape, o->boundary);
 }
 
 int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, const KCoefs& cf,
@@ -194,7 +216,8 @@ int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, 
     if (!sd || sd->dims != 3) return PERKS_ERR_INVALID_ARG;
     const int R = sd->radius;
     const bool streaming = use_stream(o, R);
-    void* fn = streaming ? stream_kernel_for(dtype, o->shape, o->boundary) : kernel_for(dtype, o->shape, o->boundary);
+    const int rows = streaming ? stream_rows(dtype, R, o) : 0;
+    void* fn = streaming ? stream_kernel_for(dtype, o->shape, o->boundary, rows) : kernel_for(dtype, o->shape, o->boundary);
     if (!fn) return PERKS_ERR_UNSUPPORTED;
     perks_device_caps caps;
     int rc = device_caps_cached(&caps);
@@ -203,7 +226,7 @@ int j3g_run(int dtype, void* d_a, void* d_b, int nz, int ny, int nx, int steps, 
     tile_of(dtype, &tx, &ty);
     if (streaming) {
         tx = 64;
-        ty = j3s::kRows;
+        ty = rows;
     }
     const int L = j3g_levels(dtype, o->shape, o);
     const size_t lds = streaming ? 0 : lds_for(dtype, R, L);

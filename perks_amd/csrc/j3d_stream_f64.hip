@@ -2,5 +2,5 @@
 #include "j3d_stream_kernel.h"
 
 namespace perks {
-void* j3s_kernel_f64(int shape, int bc) { return j3s::sptr<double>(shape, bc); }
+void* j3s_kernel_f64(int shape, int bc, int rows) { return j3s::sptr<double>(shape, bc, rows); }
 }  // namespace perks

@@ -40,7 +40,8 @@ using j3g::J3gArgs;
 using j3g::shape3;
 
 constexpr int kWaves = 4;  // waves per workgroup (each runs its own items)
-constexpr int kRows = 8;   // tile rows per wave
+constexpr int kRows = 8;   // tile rows per wave (the short tile; kRowsTall for the tall one)
+constexpr int kRowsTall = 16;
 
 // Deepest pass the kernel is built for, by radius (registers).
 __host__ __device__ constexpr int stream3_max_levels(int radius) { return radius == 1 ? 2 : 1; }
@@ -70,7 +71,7 @@ __device__ __forceinline__ double rd_lane(double v, int l)
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-template <int SHAPE, typename S, int BC, int L>
+template <int SHAPE, typename S, int BC, int L, int NY>
 struct Tile3 {
     using C = typename Fmt<S>::C;
     static constexpr const ShapeDesc& SD = shape3<SHAPE>();
@@ -78,7 +79,6 @@ struct Tile3 {
     static constexpr int Q = 2 * R + 1;  // planes of a level's window
     static constexpr int D = Q;          // level-0 planes loaded ahead
     static constexpr int UF = D;         // stages per loop trip (a multiple of Q: ring slots are static)
-    static constexpr int NY = kRows;
 
     const S* src;
     S* dst;
@@ -278,7 +278,7 @@ struct Tile3 {
     }
 };
 
-template <int SHAPE, typename S, int BC>
+template <int SHAPE, typename S, int BC, int NY>
 __global__ void __launch_bounds__(kWaves * 64) j3s_kernel(J3gArgs a, KCoefs cf)
 {
     using C = typename Fmt<S>::C;
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(kWaves * 64) j3s_kernel(J3gArgs a, KCoefs cf)
     const int W = G * kWaves;
     const int nz = a.nz, ny = a.ny, nx = a.nx;
     const int RL = R * a.levels;  // tiles use the full-pass ghost rim in every pass
-    const int VW = 64 - 2 * RL, VH = kRows - 2 * RL;
+    const int VW = 64 - 2 * RL, VH = NY - 2 * RL;
     const int ntx = (nx + VW - 1) / VW, nty = (ny + VH - 1) / VH;
     const int nitems = ntx * nty * a.nzs;
 
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(kWaves * 64) j3s_kernel(J3gArgs a, KCoefs cf)
             const int vy0 = ty * VH, vy1 = min(vy0 + VH, ny);
             auto go = [&](auto lv) {
                 constexpr int LL = decltype(lv)::value;
-                Tile3<SHAPE, S, BC, LL> t;
+                Tile3<SHAPE, S, BC, LL, NY> t;
                 t.src = src;
                 t.dst = dst;
                 t.nz = nz;
@@ -345,22 +345,35 @@ __global__ void __launch_bounds__(kWaves * 64) j3s_kernel(J3gArgs a, KCoefs cf)
     }
 }
 
-template <int SHAPE, typename S>
+template <int SHAPE, typename S, int NY>
 void* sptr_bc(int bc)
 {
-    if (bc == PERKS_BC_ZERO) return reinterpret_cast<void*>(&j3s_kernel<SHAPE, S, PERKS_BC_ZERO>);
-    return reinterpret_cast<void*>(&j3s_kernel<SHAPE, S, PERKS_BC_CLAMP>);
+    if (bc == PERKS_BC_ZERO) return reinterpret_cast<void*>(&j3s_kernel<SHAPE, S, PERKS_BC_ZERO, NY>);
+    return reinterpret_cast<void*>(&j3s_kernel<SHAPE, S, PERKS_BC_CLAMP, NY>);
 }
 
+// The tall tile is built for FP32 arithmetic (FP32, FP16, BF16 storage):
+// in FP64 its windows take every VGPR of a wave and some shapes spill.
+template <int SHAPE, typename S>
+void* sptr_rows(int bc, int rows)
+{
+    if constexpr (std::is_same<S, double>::value) {
+        return rows == kRowsTall ? nullptr : sptr_bc<SHAPE, S, kRows>(bc);
+    } else {
+        return rows == kRowsTall ? sptr_bc<SHAPE, S, kRowsTall>(bc) : sptr_bc<SHAPE, S, kRows>(bc);
+    }
+}
+
+// rows: kRows or kRowsTall tile rows per wave (nullptr: not built).
 template <typename S>
-void* sptr(int shape, int bc)
+void* sptr(int shape, int bc, int rows)
 {
     switch (shape) {
-    case PERKS_SHAPE_3D_STAR_R1: return sptr_bc<PERKS_SHAPE_3D_STAR_R1, S>(bc);
-    case PERKS_SHAPE_3D_STAR_R2: return sptr_bc<PERKS_SHAPE_3D_STAR_R2, S>(bc);
-    case PERKS_SHAPE_3D_TYPE0_R1: return sptr_bc<PERKS_SHAPE_3D_TYPE0_R1, S>(bc);
-    case PERKS_SHAPE_3D_BOX_R1: return sptr_bc<PERKS_SHAPE_3D_BOX_R1, S>(bc);
-    case PERKS_SHAPE_3D_POISSON_R1: return sptr_bc<PERKS_SHAPE_3D_POISSON_R1, S>(bc);
+    case PERKS_SHAPE_3D_STAR_R1: return sptr_rows<PERKS_SHAPE_3D_STAR_R1, S>(bc, rows);
+    case PERKS_SHAPE_3D_STAR_R2: return sptr_rows<PERKS_SHAPE_3D_STAR_R2, S>(bc, rows);
+    case PERKS_SHAPE_3D_TYPE0_R1: return sptr_rows<PERKS_SHAPE_3D_TYPE0_R1, S>(bc, rows);
+    case PERKS_SHAPE_3D_BOX_R1: return sptr_rows<PERKS_SHAPE_3D_BOX_R1, S>(bc, rows);
+    case PERKS_SHAPE_3D_POISSON_R1: return sptr_rows<PERKS_SHAPE_3D_POISSON_R1, S>(bc, rows);
     default: return nullptr;
     }
 }
@@ -368,9 +381,9 @@ void* sptr(int shape, int bc)
 }  // namespace j3s
 
 // Kernel entry points per storage format (j3d_stream_<fmt>.hip).
-void* j3s_kernel_f64(int shape, int bc);
-void* j3s_kernel_f32(int shape, int bc);
-void* j3s_kernel_f16(int shape, int bc);
-void* j3s_kernel_bf16(int shape, int bc);
+void* j3s_kernel_f64(int shape, int bc, int rows);
+void* j3s_kernel_f32(int shape, int bc, int rows);
+void* j3s_kernel_f16(int shape, int bc, int rows);
+void* j3s_kernel_bf16(int shape, int bc, int rows);
 
 }  // namespace perks

@@ -65,7 +65,7 @@ std::vector<Candidate> candidates(int dtype, const perks_stencil_opts* o)
         c.push_back({PERKS_METHOD_PERSISTENT, kResidentVariant, 1});  // resident tiles: one step per barrier
         return c;
     }
-    for (int k : {2, 3}) {  // register streaming, LDS planes
+    for (int k : {2, 4, 3}) {  // register streaming (short and tall tiles), LDS planes
         c.push_back({PERKS_METHOD_PLAIN, k, 1});
         perks_stencil_opts p = with(o, {PERKS_METHOD_PERSISTENT, k, 0});
         const int maxl = j3g_levels(dtype, o->shape, &p);

@@ -19,6 +19,7 @@ pytestmark = pytest.mark.gpu
 
 SHAPES3 = ["3d7pt", "3d13pt", "3d17pt", "3d27pt", "poisson"]
 RESIDENT = 5  # opts.tune[4]: the resident-tile form of the 7-point persistent method
+TALL = 4      # opts.tune[4], other 3D shapes: the streaming kernel's 64 x 16 tile
 DTYPES = ["f64", "f32", "f16", "bf16"]
 
 
@@ -37,6 +38,10 @@ def test_table_lists_every_configuration_and_none_spills(gpu, shape, dtype):
     if shape == "3d7pt" and dtype in ("f16", "bf16"):  # resident tiles: FP64/FP32 storage only
         assert [r["status"] for r in resident] == [L.ERR_UNSUPPORTED]
         rows = [r for r in rows if r not in resident]
+    tall = [r for r in rows if shape != "3d7pt" and r["variant"] == TALL]
+    if dtype == "f64" and shape != "3d7pt":  # the tall streaming tile: FP32 arithmetic only
+        assert tall and all(r["status"] == L.ERR_UNSUPPORTED for r in tall)
+        rows = [r for r in rows if r not in tall]
     for r in rows:
         assert r["vgprs"] > 0, r
         assert r["spill_bytes"] == 0, r
@@ -49,7 +54,7 @@ def test_table_lists_every_configuration_and_none_spills(gpu, shape, dtype):
         assert len(rows) == 1 + 4 * 5 + (1 if dtype in ("f64", "f32") else 0)
         assert sorted({r["levels"] for r in rows if r["method"] == "persistent"}) == [1, 2, 3, 4]
     else:
-        assert {r["variant"] for r in rows} == {2, 3}
+        assert {r["variant"] for r in rows} == ({2, 3} if dtype == "f64" else {2, 3, TALL})
 
 
 @pytest.mark.parametrize("dtype", ["f16", "bf16"])
