@@ -386,6 +386,12 @@ def run_j3d_storage(args):
     for name in rows:
         for m in ("persistent", "plain"):
             rows[name][m]["speedup_vs_f32"] = round(rows[name][m]["gcells_per_s"] / rows["f32"][m]["gcells_per_s"], 3)
+    # best against best: each format's faster method against FP32's faster method
+    f32_best = max(rows["f32"][m]["gcells_per_s"] for m in ("persistent", "plain"))
+    for name in rows:
+        bm = max(("persistent", "plain"), key=lambda m: rows[name][m]["gcells_per_s"])
+        rows[name]["best_method"] = bm
+        rows[name]["best_vs_f32_best"] = round(rows[name][bm]["gcells_per_s"] / f32_best, 3)
     v = rows["bf16_sr"]["plain"]
     out = {
         "metric": "GCells/s, 3d7pt 512^3 1000 steps: FP64 / FP32 / FP16 / BF16 storage, persistent and per-step, "
