@@ -47,6 +47,12 @@ Other workloads (--workload; each prints its own JSON line):
                then the 7-point solve at that height, 1000 steps, with the
                resident-tile kernel, the temporal-blocking kernel and the
                per-step kernel, compared bit for bit.
+  j2d-domain   the reference's domain selection for the 2D experiments (iteratedomain.sh /
+               iteratemindomain.sh / ProcessDomaindata.py:20-63): the per-step 2d5pt kernel
+               swept over square domains, FP32 and FP64; the chosen domain is the smallest
+               one larger than both getMinWidthY's domain and half the chip's L2 whose
+               per-step rate is within 85% of the sweep's peak; there the persistent and
+               per-step methods are timed and compared bit for bit.
   j3d-tune     the configuration table of the 3D path at 512^3 (VGPRs, spills,
                LDS, occupancy of every method / tile variant / pass depth) and
                the autotuner's timing of each, for the 7-point star in every
@@ -124,7 +130,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU sample")
     ap.add_argument("--method", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--workload", choices=["j3d", "j3d-storage", "j3d-shapes", "j2d-storage", "batch2d", "cg",
-                                                  "cg-counters", "j3d-tune", "j3d-small"],
+                                                  "cg-counters", "j3d-tune", "j3d-small", "j2d-domain"],
                     default="j3d")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--record-profile", nargs="+", metavar="DIR",
@@ -532,6 +538,68 @@ def run_j3d_small(args):
         del a, b
     print(json.dumps({"workload": "j3d-small: the 7-point solve at the getminHeight domain (zero halo, c=0.4 / 6x0.1)",
                       "unit": "GCells/s", "rows": rows, "resident_step_vs_grid": scaling}), flush=True)
+
+
+L2_BYTES = 8 * 4 * 1024 * 1024  # MI355X: 4 MB of L2 per XCD, 8 XCDs
+
+
+def run_j2d_domain(args):
+    import torch
+    import perks_amd as P
+
+    lib = P.load()
+    T = 100
+    sizes = [256, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192]
+    rows = {}
+    for dt in ("f32", "f64"):
+        esz = 4 if dt == "f32" else 8
+        tdt = torch.float32 if dt == "f32" else torch.float64
+        sweep = []
+        for n in sizes:
+            a = torch.empty(n * n, dtype=tdt, device="cuda")
+            P.fill_uniform(a, n * n, args.seed, 0.0, 1.0, dtype=dt)
+            b = torch.empty_like(a)
+            ms = []
+            for _ in range(2):
+                rep = P.jacobi2d_device(a, b, n, n, T, dtype=dt, shape="2d5pt", method="plain", boundary="clamp")
+                ms.append(rep.elapsed_ms)
+            sweep.append({"width": n, "mib": round(n * n * esz / 2 ** 20, 2),
+                          "gcells_per_s": round(T * n * n / (min(ms) / 1e3) / 1e9, 2)})
+            del a, b
+        # getMinWidthY (jacobi-2d.cu:79-364) for this width: rows whose field the chip holds
+        peak = max(r["gcells_per_s"] for r in sweep)
+        chosen = None
+        for r in sweep:
+            min_rows = lib.perks_get_min_width_y_2d(0 if dt == "f64" else 1, r["width"], 256, 256, 1, 0, 0)
+            floor = max(min_rows * r["width"] * esz, L2_BYTES // 2)
+            r["min_width_y"] = min_rows
+            if chosen is None and r["width"] * r["width"] * esz > floor and r["gcells_per_s"] >= 0.85 * peak:
+                chosen = r["width"]
+        n = chosen or sizes[-1]
+        init = torch.empty(n * n, dtype=tdt, device="cuda")
+        P.fill_uniform(init, n * n, args.seed, 0.0, 1.0, dtype=dt)
+        a, b = torch.empty_like(init), torch.empty_like(init)
+        res = {}
+        timing = {}
+        for method in ("persistent", "plain"):
+            ms = []
+            for _ in range(2):
+                a.copy_(init)
+                torch.cuda.synchronize()
+                rep = P.jacobi2d_device(a, b, n, n, 500, dtype=dt, shape="2d5pt", method=method, boundary="clamp")
+                ms.append(rep.elapsed_ms)
+            res[method] = (b if rep.result_in_b else a).view(torch.int32 if dt == "f32" else torch.int64).clone()
+            timing[method] = {"gcells_per_s": round(500 * n * n / (min(ms) / 1e3) / 1e9, 2),
+                              "ms_per_solve": round(min(ms), 3)}
+        rows[dt] = {"sweep_per_step_100_steps": sweep, "peak_gcells_per_s": peak, "chosen_width": n,
+                    "at_chosen_500_steps": timing,
+                    "speedup": round(timing["persistent"]["gcells_per_s"] / timing["plain"]["gcells_per_s"], 3),
+                    "bitwise_equal": bool(torch.equal(res["persistent"], res["plain"]))}
+        del a, b, init, res
+        torch.cuda.empty_cache()
+    print(json.dumps({"workload": "j2d-domain: 2d5pt clamp, per-step sweep over square domains, domain choice as "
+                                  "ProcessDomaindata.py, then persistent vs per-step there (500 steps)",
+                      "l2_bytes": L2_BYTES, "results": rows}), flush=True)
 
 
 def run_j3d_tune(args):
@@ -1003,7 +1071,7 @@ def main():
         sys.exit(1)
     {"j3d": run_j3d, "j3d-storage": run_j3d_storage, "j3d-shapes": run_j3d_shapes, "j2d-storage": run_j2d_storage,
      "batch2d": run_batch2d, "cg": run_cg, "cg-counters": run_cg_counters,
-     "j3d-tune": run_j3d_tune, "j3d-small": run_j3d_small}[args.workload](args)
+     "j3d-tune": run_j3d_tune, "j3d-small": run_j3d_small, "j2d-domain": run_j2d_domain}[args.workload](args)
 
 
 if __name__ == "__main__":
