@@ -22,7 +22,8 @@ int j3d_persistent_variants();
 // The form of the 7-point update a persistent run uses (j3dp::kUpd*): the
 // contracted one when opts.tune[6] = 1; the shared-product one (the gold's
 // bits) when the six neighbour weights are one positive value in the
-// arithmetic type of FP64/FP32 storage; the gold's otherwise.
+// arithmetic type (FP64, or FP32 for FP32/FP16/BF16 storage); the gold's
+// otherwise.
 int j3d_persistent_update(int dtype, const perks_stencil_opts* o, const KCoefs& cf);
 // The persistent kernel of tile variant `var` and update form `up`, and its
 // workgroup size in *block, for attribute and occupancy queries; nullptr when

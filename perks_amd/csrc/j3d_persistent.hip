@@ -221,7 +221,7 @@ int j3d_persistent_update(int dtype, const perks_stencil_opts* o, const KCoefs& 
         const double k = cf.c[0];
         same = k > 0.0;
         for (int i = 1; i < 6; ++i) same = same && cf.c[i] == k;
-    } else if (dtype == PERKS_F32) {
+    } else {  // FP32 arithmetic: FP32, FP16 and BF16 storage
         const float k = (float)cf.c[0];
         same = k > 0.0f;
         for (int i = 1; i < 6; ++i) same = same && (float)cf.c[i] == k;

@@ -1034,8 +1034,8 @@ void* pick_bc(int bc, bool has_div, int var)
 
 // sr: stochastic rounding (16-bit storage only; FP64/FP32 ignore it).
 // up: the update form (kUpd*); the fused form is built for FP64/FP32 storage
-// without a divisor, the shared-product form for FP64/FP32 storage (nullptr
-// otherwise).
+// without a divisor (nullptr otherwise), the gold and shared-product forms for
+// every storage format.
 template <int FMT>
 void* pick_kernel(int bc, bool has_div, int var, bool sr, int up)
 {
@@ -1047,7 +1047,9 @@ void* pick_kernel(int bc, bool has_div, int var, bool sr, int up)
         }
         if (up == kUpdShared) return pick_bc<FMT, false, kUpdShared>(bc, has_div, var);
     } else {
-        if (up != kUpdGold) return nullptr;
+        if (up == kUpdFused) return nullptr;
+        if (up == kUpdShared)
+            return sr ? pick_bc<FMT, true, kUpdShared>(bc, has_div, var) : pick_bc<FMT, false, kUpdShared>(bc, has_div, var);
         if (sr) return pick_bc<FMT, true, kUpdGold>(bc, has_div, var);
     }
     return pick_bc<FMT, false, kUpdGold>(bc, has_div, var);

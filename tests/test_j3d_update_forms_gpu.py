@@ -4,8 +4,9 @@ With six equal positive neighbour weights the kernel forms each product k*x
 once per cell and shares it between the terms that read x; otherwise it
 multiplies per term as the gold does.  Both must give the gold's bits, so
 each weight set below runs every tile variant at two pass depths under both
-boundaries and is compared with the CPU oracle bit for bit; the shared-product
-kernels must not spill.
+boundaries and is compared with the CPU oracle bit for bit -- FP64 and FP32
+storage, and FP16/BF16 storage (FP32 arithmetic) with both roundings; the
+shared-product kernels must not spill.
 """
 import sys
 
@@ -24,6 +25,9 @@ WEIGHTS = {
     "equal_negative": ([-0.1] * 6 + [1.6], None),         # k < 0: the gold form
     "one_differs": ([0.1] * 5 + [0.1000001, 0.4], None),  # the gold form
 }
+
+
+RESIDENT = 5  # opts.tune[4]: the resident-tile variant
 
 
 def say(*a):
@@ -51,8 +55,38 @@ def test_forms_match_oracle(gpu, dtype, boundary, weights):
             assert np.array_equal(got.view(np.uint8), want[steps].view(np.uint8)), f"variant={variant} depth={depth}"
 
 
-@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("weights", ["equal_positive", "one_differs"])
+@pytest.mark.parametrize("boundary", ["zero", "clamp"])
+@pytest.mark.parametrize("fmt", ["bf16", "f16"])
+def test_forms_16bit_storage_match_oracle(gpu, fmt, boundary, weights):
+    torch = gpu
+    coefs, div = WEIGHTS[weights]
+    st = O.stencil_from_shape(P.shape_info("3d_star_r1"), boundary, coefs=coefs, divisor=div)
+    dims, steps = (11, 70, 130), 9
+    n = int(np.prod(dims))
+    a0 = torch.empty(n, dtype=torch.int16, device="cuda")
+    P.fill_uniform(a0, n, 5, 0.0, 1.0, dtype=fmt)
+    init = a0.cpu().numpy().view(np.uint16).reshape(dims).copy()
+    want = {"nearest": O.iterate16(init, st, steps, fmt), "stochastic": O.iterate16_sr(init, st, steps, fmt, 33)}
+    for rounding in ("nearest", "stochastic"):
+        for variant in (0, 1, 2, 3, 4):
+            for depth in (3, 4):
+                say(f"[forms] {fmt} {boundary} {weights} {rounding} variant={variant} depth={depth}")
+                a = a0.clone()
+                b = torch.empty_like(a)
+                rep = P.j3d_device(a, b, *dims, steps, dtype=fmt, method="persistent", boundary=boundary, coefs=coefs,
+                                   divisor=div, rounding=rounding, round_seed=33, tune=[0, 0, depth, 0, variant])
+                got = (b if rep.result_in_b else a).cpu().numpy().view(np.uint16).reshape(dims)
+                assert np.array_equal(got, want[rounding]), f"{rounding} variant={variant} depth={depth}"
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32", "bf16", "f16"])
 def test_shared_product_kernels_do_not_spill(gpu, dtype):
-    rows = P.j3d_kernel_table(64, 64, 64, dtype=dtype, shape="3d7pt", boundary="zero", coefs=WEIGHTS["equal_positive"][0])
-    for r in rows:
-        assert r["spill_bytes"] == 0 and r["status"] == L.PERKS_OK, r
+    for rounding in (("nearest", "stochastic") if dtype in ("bf16", "f16") else ("nearest",)):
+        rows = P.j3d_kernel_table(64, 64, 64, dtype=dtype, shape="3d7pt", boundary="zero", rounding=rounding,
+                                  coefs=WEIGHTS["equal_positive"][0])
+        for r in rows:
+            if dtype in ("bf16", "f16") and r["method"] == "persistent" and r["variant"] == RESIDENT:
+                assert r["status"] == L.ERR_UNSUPPORTED, r  # the resident-tile kernel: FP64/FP32 storage
+                continue
+            assert r["spill_bytes"] == 0 and r["status"] == L.PERKS_OK, r
