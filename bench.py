@@ -152,7 +152,7 @@ def cpu_baseline(n: int, tsteps: int, dtype: str) -> dict:
     np_dt = np.float64 if dtype == "f64" else np.float32
     field = O.uniform(n ** 3, 1, 0.0, 1.0, np_dt).reshape(n, n, n)
     if O.ref_available("3d7pt_gold_gen"):
-        kind, what = "reference", "j3d_gold_iterative (stencil/3dstencil/3d7pt/3d7pt_gold_gen.cpp), g++ -O2"
+        kind, what = "reference", "j3d_gold_iterative (stencil/3dstencil/3d7pt/3d7pt_gold_gen.cpp), g++ -O3 -ffp-contract=off"
         t0 = time.perf_counter()
         O.ref_gold_iterative("3d7pt_gold_gen", field, tsteps)
         dt = time.perf_counter() - t0
@@ -171,17 +171,24 @@ def cpu_baseline(n: int, tsteps: int, dtype: str) -> dict:
         "sample": f"{what}; {n}^3 {dtype}, {tsteps} time steps, {dt:.1f} s on {os.cpu_count()} host CPUs "
                   f"({_cpu_model()}), one thread used",
     }
-    # The same sample on every host core: the oracle's OpenMP restatement
-    # (bit-identical to the single-threaded one), capped at the 16 CPUs a
-    # one-GPU share of the host provides.
-    import perks_amd as P
+    # The same sample on every host core: the gold's 7-point loop restated in
+    # the oracle with OpenMP over planes (bit-identical to the gold), capped at
+    # the 16 CPUs a one-GPU share of the host provides.
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    st = O.stencil_from_shape(P.shape_info("3d_star_r1"), "zero", coefs=CONFIG_COEFS)
-    t0 = time.perf_counter()
-    O.iterate(field, st, tsteps, threads=threads)
-    dtm = time.perf_counter() - t0
+    if dtype == "f64":
+        t0 = time.perf_counter()
+        O.star7_iterate(field, CONFIG_COEFS, "zero", tsteps, threads)
+        dtm = time.perf_counter() - t0
+        what = "oracle po_star7_iterate_omp_f64 (the gold's loop, g++ -O3 -ffp-contract=off, OpenMP over planes)"
+    else:
+        import perks_amd as P
+        st = O.stencil_from_shape(P.shape_info("3d_star_r1"), "zero", coefs=CONFIG_COEFS)
+        t0 = time.perf_counter()
+        O.iterate(field, st, tsteps, threads=threads)
+        dtm = time.perf_counter() - t0
+        what = "oracle po_stencil_iterate_omp_f32"
     out["all_cores"] = {"value": round(tsteps * n ** 3 / dtm / 1e9, 4), "unit": "GCells/s", "cores": threads,
-                        "kind": "port", "sample": f"oracle po_stencil_iterate_omp ({threads} threads); {n}^3 {dtype}, "
+                        "kind": "port", "sample": f"{what}, {threads} threads; {n}^3 {dtype}, "
                                                   f"{tsteps} time steps, {dtm:.2f} s"}
     return out
 

@@ -1,7 +1,7 @@
 /*
  * perks_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY; see perks_oracle.h).
  *
- * Build: oracle/Makefile, `-O2 -ffp-contract=off` with no FMA target flags so
+ * Build: oracle/Makefile, `-O3 -ffp-contract=off` with no FMA target flags so
  * every product and sum is rounded exactly once, in the order written.
  */
 #include "perks_oracle.h"
@@ -217,6 +217,73 @@ int po_stencil_iterate_fused_##SUF(const T* in, T* out, int nz, int ny, int nx, 
 
 DEFINE_FUSED(f64, double, fma)
 DEFINE_FUSED(f32, float, fmaf)
+
+/* 7-point star, the gold's loop (3d7pt_gold_gen.cpp:9-40) with the edge
+ * columns split off so the interior loop has no boundary tests.  Rows and
+ * planes beyond a face are `zrow` (zeros) under the zero boundary and the
+ * edge row or plane under clamp. */
+static void star7_plane(const double* in, double* out, int nz, int ny, int nx, const double* k, int clamp,
+                        const double* zrow, int z)
+{
+    const size_t plane = (size_t)ny * nx;
+    const int zt = z + 1 < nz ? z + 1 : (clamp ? nz - 1 : -1);
+    const int zb = z > 0 ? z - 1 : (clamp ? 0 : -1);
+    for (int y = 0; y < ny; ++y) {
+        const int yn = y + 1 < ny ? y + 1 : (clamp ? ny - 1 : -1);
+        const int ys = y > 0 ? y - 1 : (clamp ? 0 : -1);
+        const double* c = in + (size_t)z * plane + (size_t)y * nx;
+        const double* N = yn < 0 ? zrow : in + (size_t)z * plane + (size_t)yn * nx;
+        const double* S = ys < 0 ? zrow : in + (size_t)z * plane + (size_t)ys * nx;
+        const double* T = zt < 0 ? zrow : in + (size_t)zt * plane + (size_t)y * nx;
+        const double* B = zb < 0 ? zrow : in + (size_t)zb * plane + (size_t)y * nx;
+        double* o = out + (size_t)z * plane + (size_t)y * nx;
+        for (int side = 0; side < (nx > 1 ? 2 : 1); ++side) {  /* the two edge columns */
+            const int x = side ? nx - 1 : 0;
+            const double e = x + 1 < nx ? c[x + 1] : (clamp ? c[nx - 1] : 0.0);
+            const double w = x > 0 ? c[x - 1] : (clamp ? c[0] : 0.0);
+            double acc = k[0] * e;
+            acc = acc + k[1] * w;
+            acc = acc + k[2] * N[x];
+            acc = acc + k[3] * S[x];
+            acc = acc + k[4] * T[x];
+            acc = acc + k[5] * B[x];
+            acc = acc + k[6] * c[x];
+            o[x] = acc;
+        }
+        for (int x = 1; x < nx - 1; ++x) {
+            double acc = k[0] * c[x + 1];
+            acc = acc + k[1] * c[x - 1];
+            acc = acc + k[2] * N[x];
+            acc = acc + k[3] * S[x];
+            acc = acc + k[4] * T[x];
+            acc = acc + k[5] * B[x];
+            acc = acc + k[6] * c[x];
+            o[x] = acc;
+        }
+    }
+}
+
+void po_star7_iterate_omp_f64(const double* in, double* out, int nz, int ny, int nx, const double* k, int boundary,
+                              int steps, int threads)
+{
+    const size_t n = (size_t)nz * ny * nx;
+    if (steps <= 0) { memcpy(out, in, n * sizeof(double)); return; }
+    const int clamp = boundary == PO_BC_CLAMP;
+    double* zrow = (double*)calloc((size_t)nx, sizeof(double));
+    /* the parity rule of iterate_*: the last step writes `out` */
+    double* tmp = (double*)malloc(n * sizeof(double));
+    double* dst = (steps % 2 == 1) ? out : tmp;
+    double* other = (steps % 2 == 1) ? tmp : out;
+    const double* src = in;
+    for (int s = 0; s < steps; ++s) {
+        _Pragma("omp parallel for num_threads(threads) schedule(static)")
+        for (int z = 0; z < nz; ++z) star7_plane(src, dst, nz, ny, nx, k, clamp, zrow, z);
+        src = dst;
+        double* t = dst; dst = other; other = t;
+    }
+    free(tmp);
+    free(zrow);
+}
 
 /* ------------------------------------------------------------------------ */
 /* Reduced-precision storage                                                 */

@@ -74,6 +74,13 @@ void po_stencil_iterate_f32(const float* in, float* out, int nz, int ny, int nx,
 /* Same, with the z loop split across `threads` OpenMP threads (bitwise equal). */
 void po_stencil_iterate_omp_f64(const double* in, double* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
 void po_stencil_iterate_omp_f32(const float* in, float* out, int nz, int ny, int nx, const po_stencil* st, int steps, int threads);
+/* The 3D 7-point star as the gold's own loop (3d7pt_gold_gen.cpp:9-40): one
+ * cell at a time, terms e, w, n, s, t, b, c with weights k[0..6], each product
+ * and sum rounded once; zero or clamp boundary; z planes split across
+ * `threads` OpenMP threads.  Bitwise equal to po_stencil_iterate_f64 with
+ * the same weights (the CPU baseline's all-cores figure times it). */
+void po_star7_iterate_omp_f64(const double* in, double* out, int nz, int ny, int nx, const double* k, int boundary,
+                              int steps, int threads);
 /* Contracted form (no reference counterpart): group sums as above, the first
  * product rounded alone, each later one fused, acc = fma(coef, s, acc).
  * Returns -1 (nothing written) for FP32 stencils with double-formed groups. */

@@ -52,6 +52,7 @@ def lib() -> C.CDLL:
             "po_stencil_iterate_f32": (None, [P, P, I, I, I, P, I]),
             "po_stencil_iterate_omp_f64": (None, [P, P, I, I, I, P, I, I]),
             "po_stencil_iterate_omp_f32": (None, [P, P, I, I, I, P, I, I]),
+            "po_star7_iterate_omp_f64": (None, [P, P, I, I, I, P, I, I, I]),
             "po_stencil_iterate_fused_f64": (I, [P, P, I, I, I, P, I]),
             "po_stencil_iterate_fused_f32": (I, [P, P, I, I, I, P, I]),
             "po_f32_to_f16": (C.c_uint16, [C.c_float]),
@@ -143,6 +144,19 @@ def iterate(field: np.ndarray, st: PoStencil, steps: int, threads: int = 1) -> n
     else:
         fn = lib().po_stencil_iterate_f64 if f64 else lib().po_stencil_iterate_f32
         fn(p(field), p(out), nz, ny, nx, C.byref(st), steps)
+    return out
+
+
+def star7_iterate(field: np.ndarray, k, boundary: str, steps: int, threads: int = 1) -> np.ndarray:
+    """The 3D 7-point star as the gold's own loop (po_star7_iterate_omp_f64):
+    weights k in the term order e, w, n, s, t, b, c."""
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    nz, ny, nx = field.shape
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    assert kk.shape == (7,)
+    out = np.empty_like(field)
+    lib().po_star7_iterate_omp_f64(p(field), p(out), nz, ny, nx, p(kk), 1 if boundary == "zero" else 0, steps,
+                                   max(1, threads))
     return out
 
 
