@@ -121,7 +121,8 @@ typedef struct perks_stencil_opts {
      *           sum by an FMA; not the gold's bits, within a few ulp of them per step);
      *           PERKS_ERR_UNSUPPORTED for any other method, shape, format or a divisor.
      *           0 (default) = uncontracted, bit-identical to the reference CPU gold
-     *   tune[7] unused */
+     *   tune[7] 3D 7-point persistent (ring kernel): z-slabs per x-y tile, each one a
+     *           work item of a pass (capped at the height); 0 = the planner's choice */
     int32_t tune[8];
     /* FP16/BF16 storage only (no reference counterpart): how each step's FP32
      * result is rounded to the storage format.  PERKS_ROUND_NEAREST (default):

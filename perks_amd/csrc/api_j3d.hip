@@ -116,6 +116,7 @@ int validate(const T* a, const T* b, int nz, int ny, int nx, int steps, const pe
         if (o->tune[6] != 1) return PERKS_ERR_INVALID_ARG;
         if (sizeof(T) == 2 || uses_general(o) || o->method != PERKS_METHOD_PERSISTENT) return PERKS_ERR_UNSUPPORTED;
     }
+    if (o->tune[7] < 0) return PERKS_ERR_INVALID_ARG;  // z-slabs per tile (0: the planner's)
     return PERKS_OK;
 }
 

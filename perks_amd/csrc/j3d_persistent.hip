@@ -159,7 +159,7 @@ int plan_persistent(int dtype, int nz, int ny, int nx, const perks_stencil_opts*
         // x tiles as the kernel cuts them (j3dp::x_tiles; the two-column form
         // keeps ghost lanes at both faces)
         const int ntx = v.pair ? (nx + tile_width(v, l) - 1) / tile_width(v, l) : j3dp::x_tiles(o->boundary, nx, l);
-        out->reserved[l] = best_nzs(nz, ny, ntx, l, G, v.nw * v.ny);
+        out->reserved[l] = o->tune[7] > 0 ? std::min(o->tune[7], nz) : best_nzs(nz, ny, ntx, l, G, v.nw * v.ny);
     }
     return PERKS_OK;
 }
