@@ -106,18 +106,21 @@ template <>
 __device__ __forceinline__ uint32_t narrow_sr<PERKS_BF16>(float v, uint32_t r)
 {
     const uint32_t u = __float_as_uint(v);
-    return ((u & 0x7FFFFFFFu) > 0x7F800000u) ? ((u >> 16) | 0x40u) : ((u + (r & 0xFFFFu)) >> 16);
+    return __builtin_isnan(v) ? ((u >> 16) | 0x40u) : ((u + (r & 0xFFFFu)) >> 16);
 }
 
+// FP16: an exponent field of 113..254 is the FP16 normal range and above.
+// There the truncated sum t has 13 zero low bits, so converting it is exact,
+// or gives +-inf once |t| > 65504 (the next such value, 65536, rounds to inf):
+// the overflow rule above by the conversion itself.  Other inputs (FP16
+// subnormal results, zero, inf, NaN) are converted from v, to nearest even.
 template <>
 __device__ __forceinline__ uint32_t narrow_sr<PERKS_F16>(float v, uint32_t r)
 {
     const uint32_t u = __float_as_uint(v);
     const uint32_t e = (u >> 23) & 0xFFu;
     const uint32_t t = (u + (r & 0x1FFFu)) & 0xFFFFE000u;
-    const uint32_t sr = ((t & 0x7FFFFFFFu) > 0x477FE000u) ? (((u >> 16) & 0x8000u) | 0x7C00u)
-                                                          : Store16<PERKS_F16>::narrow(__uint_as_float(t));
-    return (e < 113u || e == 255u) ? Store16<PERKS_F16>::narrow(v) : sr;
+    return Store16<PERKS_F16>::narrow(e - 113u <= 141u ? __uint_as_float(t) : v);
 }
 
 // Rounds v to the storage format: nearest even, or stochastically with the
