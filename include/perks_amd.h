@@ -105,7 +105,7 @@ typedef struct perks_stencil_opts {
      *           (3D 7-point default: 4; under the clamp boundary 3 for FP16 and for BF16
      *           with nearest rounding; perks_j3d_autotune measures every choice)
      *   tune[4] tile variant.  3D 7-point persistent: 0..3 (rows x waves of its kernel;
-     *           FP64 0 = 8 waves x 5 rows), 4 = two columns per lane (FP32/FP16/BF16, even
+     *           FP64 0 = 12 waves x 3 rows, 3 = 8 waves x 5 rows), 4 = two columns per lane (FP32/FP16/BF16, even
      *           widths) or, for FP64, 8 waves x 4 rows; 5 = resident tiles (FP64/FP32): the whole domain held
      *           in VGPRs across every step, one grid barrier per step, halos only through
      *           memory; PERKS_ERR_UNSUPPORTED_SIZE above perks_j3d_min_height's height.

@@ -975,22 +975,24 @@ j3d_star1_persistent_kernel(typename Elem<FMT>::S* buf_a, typename Elem<FMT>::S*
 // count, so the tables differ for FP64 and for FP32 arithmetic.  Variant 0 is
 // the default (the fastest measured at 512^3):
 //   FP32 arithmetic (FP32, FP16, BF16 storage)    FP64
-//   0  16 waves x 4 rows (64 x 64 tile)            8 waves x 5 rows (64 x 40 tile)
+//   0  16 waves x 4 rows (64 x 64 tile)            12 waves x 3 rows, 5 slots (64 x 36)
 //   1  8 waves x 4 rows, 9 slots                  8 waves x 4 rows, 9 slots
 //   2  8 waves x 4 rows (two workgroups per CU)   4 waves x 4 rows (two per CU)
-//   3  16 waves x 2 rows                          16 waves x 2 rows
+//   3  16 waves x 2 rows                          8 waves x 5 rows (64 x 40 tile)
 //   4  8 waves x 4 rows, two columns per lane     8 waves x 4 rows (64 x 32 tile)
 //      (FP32: a 128 x 32 tile; even grid widths; at 512^3 25% slower than 0
-//      for FP32: twice the registers halve the occupancy.  FP64 8 x 5 rows:
-//      ~230 VGPRs, 3% faster than 8 x 4 at 512^3 and 6% at 384^3 with the
-//      y edge tiles; 6 rows would need more than 256 VGPRs and spill)
+//      for FP32: twice the registers halve the occupancy.  FP64: the 12-wave
+//      workgroup needs three waves per SIMD, so its allocation stays within
+//      168 VGPRs (five level-0 slots; six would spill): 2.7% faster than
+//      8 x 5 rows at 512^3, whose ~230 VGPRs allow two waves per SIMD; 8 x 5
+//      is 3% faster than 8 x 4; 6 rows would need more than 256 VGPRs)
 // Selected through opts->tune[4].
 struct Variant {
     int ny, nw, p, wpe;
     int pair;  // two columns per lane (March2)
 };
 constexpr Variant kVarF32[] = {{4, 16, 6, 1, 0}, {4, 8, 9, 1, 0}, {4, 8, 6, 1, 0}, {2, 16, 6, 1, 0}, {4, 8, 6, 1, 1}};
-constexpr Variant kVarF64[] = {{5, 8, 6, 1, 0}, {4, 8, 9, 1, 0}, {4, 4, 6, 1, 0}, {2, 16, 6, 1, 0}, {4, 8, 6, 1, 0}};
+constexpr Variant kVarF64[] = {{3, 12, 5, 1, 0}, {4, 8, 9, 1, 0}, {4, 4, 6, 1, 0}, {5, 8, 6, 1, 0}, {4, 8, 6, 1, 0}};
 constexpr int kNumVariants = 5;
 constexpr int kPairVariant = 4;
 __host__ __device__ constexpr Variant variant_shape(int fmt, int v)

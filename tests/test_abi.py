@@ -92,7 +92,7 @@ def caps(cu=256):
 
 def test_planner_is_host_only_and_refuses_like_the_reference():
     rc, info = P.plan(512, 512, 512, caps=caps(), occupancy=1, method="persistent")
-    assert rc == 0 and info.grid[0] == 256 and info.block == 512
+    assert rc == 0 and info.grid[0] == 256 and info.block == 12 * 64  # FP64 default: 12 waves x 3 rows
     rc, _ = P.plan(512, 512, 512, caps=caps(), occupancy=0, method="persistent")
     assert rc == L.ERR_NOT_LAUNCHABLE                       # j3d.cu:570
     rc, info = P.plan(64, 64, 64, caps=caps(), occupancy=1, method="persistent", shape="3d27pt")
