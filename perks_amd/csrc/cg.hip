@@ -1023,6 +1023,20 @@ const void* kernel_ptr()
 // matrix; the occupancy query with that dynamic LDS must still admit the
 // grid's workgroups per CU, else the budget shrinks and the plan is redone
 // (cg_dispatcher.cuh:688-748 sizes the reference's caches the same way).
+// LDS is allocated per workgroup in whole granules (static and dynamic
+// together), so a share of exactly lds_per_cu / wpc can round up past the
+// CU's LDS: then only wpc - 1 workgroups fit per CU, part of the cooperative
+// grid never starts, and the first grid barrier waits for it until its time
+// limit (at 256^3 and three workgroups per CU: 54384 B per workgroup, 54784
+// in 512-byte granules, three of them 164352 B > 163840).  The share is
+// rounded down to a multiple of every plausible granule (512 B, 1 KiB,
+// 1280 B, 2 KiB), so wpc workgroups fit whichever the hardware uses.
+static long long lds_share(long long per_wg)
+{
+    const long long a = per_wg / 2048 * 2048, b = per_wg / 1280 * 1280;
+    return a < b ? a : b;
+}
+
 template <typename V, int FMT>
 int plan_launch(const CgArgs<V>& a, const perks_cg_opts* o, int* grid, CgCache* cache)
 {
@@ -1044,8 +1058,8 @@ int plan_launch(const CgArgs<V>& a, const perks_cg_opts* o, int* grid, CgCache* 
     PERKS_HIP_TRY(hipFuncGetAttributes(&fa, fn));
     const int rounds = (a.nseg + G - 1) / G;
     const long long table = (long long)slot_table_bytes(rounds);
-    long long budget =
-        std::min<long long>(caps.lds_per_block, caps.lds_per_cu / wpc) - (long long)fa.sharedSizeBytes - table;
+    long long budget = lds_share(std::min<long long>(caps.lds_per_block, caps.lds_per_cu / wpc)) -
+                       (long long)fa.sharedSizeBytes - table;
     budget = budget / 256 * 256;
     const long long unit = (long long)kLanes * sizeof(V);
     const long long share = (long long)((a.nseg + G - 1) / G) * kPerLane;  // 256-row units of a workgroup
@@ -1072,7 +1086,7 @@ int plan_launch(const CgArgs<V>& a, const perks_cg_opts* o, int* grid, CgCache* 
         if (c.bytes > 65536 &&
             hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.bytes) != hipSuccess) {
             (void)hipGetLastError();
-            budget = 65536;
+            budget = (lds_share(65536) - (long long)fa.sharedSizeBytes - table) / 256 * 256;
             continue;
         }
         PERKS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, fn, kLanes, (size_t)c.bytes));

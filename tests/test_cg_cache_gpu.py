@@ -191,3 +191,38 @@ def test_poisson_256_full_configuration(gpu):
         torch.cuda.empty_cache()
     assert len(hist["csr"]) > 500
     assert np.array_equal(hist["csr"], hist["dia"])
+
+
+@pytest.mark.parametrize("fmt", ["csr", "dia"])
+def test_every_workgroups_per_cu_fits_at_256(gpu, fmt):
+    """256^3 with 1 to 4 workgroups per CU (opts.blkpsm): the planner sizes each
+    workgroup's LDS caches to the CU's LDS divided by that count, rounded to the
+    allocation granule, so the whole cooperative grid is resident at every
+    count (three per CU once took 54784 B each in 512-byte granules, more than
+    a CU holds, and the grid barrier timed out).  Every count gives the same
+    residual history."""
+    torch = gpu
+    m = 256
+    n = m ** 3
+    b = torch.empty(n, dtype=torch.float64, device="cuda")
+    P.fill_uniform(b, n, 1, -1.0, 1.0)
+    if fmt == "csr":
+        nnz = P.poisson7_nnz(m, m, m)
+        rowptr = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+        col = torch.empty(nnz, dtype=torch.int32, device="cuda")
+        val = torch.empty(nnz, dtype=torch.float64, device="cuda")
+        P.poisson7_csr(m, m, m, rowptr, col, val)
+        mat = dict(rowptr=rowptr, col=col, val=val, nnz=nnz)
+    else:
+        val = torch.empty(7 * n, dtype=torch.float64, device="cuda")
+        mat = dict(offsets=P.poisson7_dia(m, m, m, val), val=val)
+    ref = None
+    for blk in (4, 3, 2, 1):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        say(f"[cgc] 256^3 {fmt} blkpsm={blk}")
+        res = P.cg_solve(fmt, n, b, x, tol=1e-10, max_iter=5000, x0_is_zero=True, blkpsm=blk, **mat)
+        say(f"[cgc]   grid {res.grid}, {res.iterations} iterations, {res.elapsed_ms:.1f} ms, LDS {res.lds_bytes} B")
+        assert res.code == 0 and res.grid == 256 * blk and res.relative_residual < 1e-10
+        if ref is None:
+            ref = res.history
+        assert np.array_equal(res.history, ref), blk
